@@ -1,0 +1,34 @@
+// Device math probe: lets the tests compare the stages' sqrt, exp and division
+// bits with the host's.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "clue_amd.h"
+#include "clue_device.h"
+
+namespace {
+
+__global__ void k_debug_math(int n, const float* __restrict__ a, const float* __restrict__ b,
+                             float* __restrict__ out_sqrt, float* __restrict__ out_exp,
+                             float* __restrict__ out_div) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i];
+  if (out_sqrt) out_sqrt[i] = __builtin_sqrtf(x);
+  if (out_exp) out_exp[i] = clue_dev::expf_cw(x);
+  if (out_div) out_div[i] = x / b[i];
+}
+
+}  // namespace
+
+extern "C" clue_status clue_debug_math_impl(void* stream, int32_t n, const float* a,
+                                            const float* b, float* out_sqrt, float* out_exp,
+                                            float* out_div) {
+  if (n <= 0) return CLUE_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, s, n, a, b, out_sqrt,
+                     out_exp, out_div);
+  if (hipGetLastError() != hipSuccess) return CLUE_ERR_HIP;
+  return hipStreamSynchronize(s) == hipSuccess ? CLUE_OK : CLUE_ERR_HIP;
+}

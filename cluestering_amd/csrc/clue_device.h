@@ -1,0 +1,128 @@
+// Device-side building blocks shared by the CLUE stages: wave-level primitives,
+// the exp restatement, tile geometry and distance metrics. gfx950 only.
+//
+// FP contract: all device code is compiled with -ffp-contract=off, and every
+// expression below is evaluated in the order the reference writes it, so that
+// results are bit-identical to the serial CPU semantics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "clue_types.h"
+
+namespace clue_dev {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Wave primitives (wave64).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Peers: the set of active lanes whose key equals this lane's key, built from one
+// ballot per key bit. Lanes with valid == false are excluded.
+__device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool valid) {
+  uint64_t peers = ballot(valid);
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = (key >> b) & 1u;
+    const uint64_t m = ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return valid ? peers : 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// exp: Cody-Waite reduction x = k ln2 + r, degree-7 Taylor polynomial of e^r in
+// Horner form with fused multiply-adds, scaling by 2^k. Same method, constants
+// and operation order as the CPU statement of the algorithm, so both give the
+// same bits.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float expf_cw(float x) {
+  if (x != x) return x;
+  if (x > 0x1.62e42ep+6f) return __builtin_inff();
+  if (x < -0x1.9fe368p+6f) return 0.0f;
+  const float log2e = 0x1.715476p+0f;
+  const float ln2_hi = 0x1.62e400p-1f;
+  const float ln2_lo = 0x1.7f7d1cp-20f;
+  const float k = __builtin_rintf(x * log2e);
+  float r = __builtin_fmaf(-k, ln2_hi, x);
+  r = __builtin_fmaf(-k, ln2_lo, r);
+  float p = 0x1.a01a02p-13f;
+  p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+  p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+  p = __builtin_fmaf(p, r, 0x1.000000p-1f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  return __builtin_ldexpf(p, static_cast<int>(k));
+}
+
+// static_cast<int>(float) with the x86 semantics the reference's serial backend
+// runs with (cvttss2si): NaN and out-of-range values give INT_MIN.
+__device__ __forceinline__ int cvt_trunc(float q) {
+  if (!(q >= -2147483648.0f) || !(q < 2147483648.0f)) return INT32_MIN;
+  return static_cast<int>(q);
+}
+
+// Tile geometry of one event (TilesView.hpp:38-104).
+struct Geo {
+  int D;
+  int nper;
+  const float* mn;
+  const float* mx;
+  const float* side;
+  const int32_t* wrapped;
+};
+
+__device__ __forceinline__ float normalize(const Geo& g, float c, int d) {
+  const float range = g.mx[d] - g.mn[d];
+  float r = c - static_cast<float>(cvt_trunc(c / range)) * range;
+  if (r >= g.mx[d])
+    r -= range;
+  else if (r < g.mn[d])
+    r += range;
+  return r;
+}
+
+__device__ __forceinline__ int get_bin(const Geo& g, float c, int d) {
+  int b;
+  if (g.wrapped[d])
+    b = cvt_trunc((normalize(g, c, d) - g.mn[d]) / g.side[d]);
+  else
+    b = cvt_trunc((c - g.mn[d]) / g.side[d]);
+  b = min(b, g.nper - 1);
+  b = max(b, 0);
+  return b;
+}
+
+// Ordered-integer encoding of floats, so that integer min/max equal float min/max.
+__device__ __forceinline__ int32_t float_to_ordered(float f) {
+  const int32_t b = __float_as_int(f);
+  return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+__device__ __forceinline__ float ordered_to_float(int32_t o) {
+  return __int_as_float(o >= 0 ? o : (o ^ 0x7fffffff));
+}
+
+}  // namespace clue_dev

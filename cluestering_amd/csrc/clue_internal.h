@@ -1,0 +1,106 @@
+// Host-side declarations of the device stages and primitives.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "clue_types.h"
+
+namespace clue_dev {
+
+// ---- primitives (clue_primitives.hip) ----
+int64_t scan_temp_elems(int64_t n);
+// Exclusive scan of n int32 (in may equal out). temp: scan_temp_elems(n) ints.
+// total (device, may be NULL) receives the sum.
+void scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* temp, int32_t* total,
+                    hipStream_t s);
+int64_t radix_hist_elems(int64_t n);
+void radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n, int key_bits,
+                      int32_t* hist, int32_t* scan_temp, bool* in_first, hipStream_t s);
+
+// ---- stages (clue_stages.hip) ----
+
+// Input columns and the sorted copies, all device pointers.
+struct PointArrays {
+  const float* x[kMaxDim];
+  const float* w;
+  const float* sigma[kMaxDim];
+  const float* u;
+  const uint32_t* tags;
+};
+
+struct SortedArrays {
+  float* x[kMaxDim];
+  float* w;
+  float* sigma[kMaxDim];
+  float* u;
+  uint32_t* tag;  // user tags, or the original index when no tags are given
+  int32_t* perm;  // sorted position -> original index
+};
+
+// Everything one batch run needs on the device.
+struct BatchBuffers {
+  int32_t n_events;
+  int32_t n_total;
+  int32_t n_chunks;
+  int32_t n_tiles_total;  // sum of ntiles over events
+  int32_t max_items;      // upper bound of work items
+  int32_t key_bits;
+  const EventDesc* events;
+  const Chunk* chunks;
+  int32_t* ext;    // [E][D][2] ordered-int extremes
+  float* geo;      // [E][3][D] mn, mx, side
+  uint32_t* keys0;
+  int32_t* vals0;
+  uint32_t* keys1;
+  int32_t* vals1;
+  int32_t* rs_hist;
+  int32_t* scan_tmp;
+  int32_t* toff;       // [n_tiles_total + 1]
+  int32_t* icount;     // [n_tiles_total]
+  int32_t* n_items;    // [1]
+  Item* items;         // [max_items]
+  float* srho;
+  int32_t* snh;
+  float* sdelta;
+  int32_t* flag;       // [n_total]
+  int32_t* rank;       // [n_total]
+  int32_t* n_seeds;    // [1]
+  int32_t* seeds;      // [n_total]
+  SortedArrays sorted;
+  int64_t sorted_stride;  // column stride of sorted.x[] and sorted.sigma[] blocks
+};
+
+struct ResultArrays {
+  int32_t* cluster;
+  int32_t* is_seed;
+  float* rho;
+  int32_t* nh;
+  float* delta;
+  int32_t* event_of_cluster;  // may be NULL
+};
+
+// Stage boundaries reported by the profiler.
+enum Stage {
+  kStageExtremes = 0,
+  kStageTiles,
+  kStageDensity,
+  kStageNearestHigher,
+  kStageSeeds,
+  kStageLabels,
+  kNumStages
+};
+const char* stage_name(int s);
+
+// Timing hook called between stages (may be a no-op).
+struct StageHook {
+  void (*mark)(void* self, int stage, hipStream_t s);
+  void* self;
+};
+
+// Launches the whole pipeline for one batch on stream s. No host synchronisation,
+// no allocation, no memset: safe to capture in a hipGraph.
+void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuffers& bb,
+                     const ResultArrays& out, hipStream_t s, const StageHook* hook);
+
+}  // namespace clue_dev

@@ -1,0 +1,699 @@
+// The five CLUE stages on gfx950:
+//   (a) tile construction: per-event extremes, tile keys, stable radix sort into a
+//       tile-sorted SoA copy, tile offsets, work items;
+//   (b) local density and (c) nearest-higher: one wave per work item (up to 64
+//       points of one tile, one lane each) walks the union of the lanes' search
+//       boxes in the reference's lexicographic tile order; candidates are
+//       wave-uniform and come through the scalar cache;
+//   (d) seeds, numbered by an exclusive scan in input order;
+//   (e) follower labels.
+//
+// Semantics follow the reference's serial backend:
+//   geometry  core/detail/SetupTiles.hpp:23-57, core/detail/ComputeTiles.hpp:17-37
+//   bins/box  data_structures/internal/TilesView.hpp:38-104
+//   density   core/detail/ClusteringKernels.hpp:29-140
+//   NH        core/detail/ClusteringKernels.hpp:142-291
+//   seeds     core/detail/ClusteringKernels.hpp:293-343
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "clue_device.h"
+#include "clue_internal.h"
+
+namespace clue_dev {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+constexpr float kSqrt2 = 0x1.6a09e6p+0f;  // sqrtf(2.0f), correctly rounded
+
+enum { M_EUCLID = 0, M_WEUCLID = 1, M_PERIODIC = 2, M_MANHATTAN = 3, M_CHEBYSHEV = 4,
+       M_WCHEBYSHEV = 5, M_MAHALANOBIS = 6 };
+enum { K_FLAT = 0, K_EXP = 1, K_GAUSS = 2 };
+
+template <int M>
+struct MetricTraits {
+  // Metrics ending in a square root are compared through the squared sum against
+  // the largest float whose correctly rounded square root is within the radius.
+  static constexpr bool kSqrt =
+      (M == M_EUCLID || M == M_WEUCLID || M == M_PERIODIC || M == M_MAHALANOBIS);
+};
+
+// ---------------------------------------------------------------------------
+// Utility kernels.
+// ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// (a1) Per-event coordinate extremes (ComputeTiles.hpp:17-36). NaN coordinates
+// are skipped, as the reference's std::max/std::min fold skips them.
+// ---------------------------------------------------------------------------
+struct CoordPtrs {
+  const float* x[kMaxDim];
+};
+
+__global__ __launch_bounds__(kBlock) void k_extremes(CoordPtrs c, int D,
+                                                     const Chunk* __restrict__ chunks,
+                                                     int32_t* __restrict__ ext) {
+  const Chunk ch = chunks[blockIdx.x];
+  __shared__ int32_t smin[kWaves][kMaxDim], smax[kWaves][kMaxDim];
+  const int w = threadIdx.x / kWave;
+  for (int d = 0; d < D; ++d) {
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
+      const float v = c.x[d][i];
+      if (v != v) continue;
+      const int32_t o = float_to_ordered(v);
+      lo = min(lo, o);
+      hi = max(hi, o);
+    }
+    lo = wave_min_i(lo);
+    hi = wave_max_i(hi);
+    if (lane_id() == 0) {
+      smin[w][d] = lo;
+      smax[w][d] = hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < D) {
+    const int d = threadIdx.x;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int k = 0; k < kWaves; ++k) {
+      lo = min(lo, smin[k][d]);
+      hi = max(hi, smax[k][d]);
+    }
+    if (lo != INT32_MAX) atomicMin(&ext[(ch.event * D + d) * 2 + 0], lo);
+    if (hi != INT32_MIN) atomicMax(&ext[(ch.event * D + d) * 2 + 1], hi);
+  }
+}
+
+// Extremes start from the identities of min (INT32_MAX) and max (INT32_MIN).
+__global__ void k_init_extremes(int32_t* __restrict__ ext, int64_t pairs) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < pairs) {
+    ext[2 * k] = INT32_MAX;
+    ext[2 * k + 1] = INT32_MIN;
+  }
+}
+
+// geo[e] = {mn[D], mx[D], side[D]}, side = (max - min) / nper (ComputeTiles.hpp:33-34).
+__global__ void k_geometry(const int32_t* __restrict__ ext, const EventDesc* __restrict__ events,
+                           int n_events, int D, float* __restrict__ geo) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_events) return;
+  const int nper = events[e].nper;
+  for (int d = 0; d < D; ++d) {
+    const int32_t lo = ext[(e * D + d) * 2 + 0];
+    const int32_t hi = ext[(e * D + d) * 2 + 1];
+    const float mn = lo == INT32_MAX ? FLT_MAX : ordered_to_float(lo);
+    const float mx = hi == INT32_MIN ? -FLT_MAX : ordered_to_float(hi);
+    float* g = geo + static_cast<int64_t>(e) * 3 * D;
+    g[d] = mn;
+    g[D + d] = mx;
+    g[2 * D + d] = (mx - mn) / static_cast<float>(nper);
+  }
+}
+
+__device__ __forceinline__ Geo event_geo(const float* geo, const RunParams& rp, int e, int nper) {
+  Geo g;
+  g.D = rp.D;
+  g.nper = nper;
+  g.mn = geo + static_cast<int64_t>(e) * 3 * rp.D;
+  g.mx = g.mn + rp.D;
+  g.side = g.mn + 2 * rp.D;
+  g.wrapped = rp.wrapped;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// (a2) Tile key of every point: t_base + sum_d nper^(D-1-d) * bin_d
+// (TilesView.hpp:54-64), and the identity permutation.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_tile_keys(CoordPtrs c, const RunParams rp,
+                                                      const Chunk* __restrict__ chunks,
+                                                      const EventDesc* __restrict__ events,
+                                                      const float* __restrict__ geo,
+                                                      uint32_t* __restrict__ keys,
+                                                      int32_t* __restrict__ vals) {
+  const Chunk ch = chunks[blockIdx.x];
+  const EventDesc ev = events[ch.event];
+  const Geo g = event_geo(geo, rp, ch.event, ev.nper);
+  for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
+    int t = 0;
+    for (int d = 0; d < rp.D; ++d) t = t * ev.nper + get_bin(g, c.x[d][i], d);
+    keys[i] = static_cast<uint32_t>(ev.t_base + t);
+    vals[i] = i;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (a3) Gather the tile-sorted SoA copy and the tile offsets. toff[k] is the first
+// sorted position whose key is >= k, so tile k spans [toff[k], toff[k+1]).
+// ---------------------------------------------------------------------------
+struct GatherArgs {
+  PointArrays in;
+  SortedArrays out;
+  int64_t cs;  // column stride of the sorted coordinate block
+};
+
+__global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs a, const RunParams rp,
+                                                   const uint32_t* __restrict__ keys,
+                                                   const int32_t* __restrict__ perm, int n,
+                                                   int n_tiles_total, int32_t* __restrict__ toff) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  for (int d = 0; d < rp.D; ++d) a.out.x[d][s] = a.in.x[d][i];
+  a.out.w[s] = a.in.w[i];
+  if (rp.has_sigma)
+    for (int d = 0; d < rp.D; ++d) a.out.sigma[d][s] = a.in.sigma[d][i];
+  if (rp.has_u) a.out.u[s] = a.in.u[i];
+  a.out.tag[s] = rp.has_tags ? a.in.tags[i] : static_cast<uint32_t>(i);
+  a.out.perm[s] = i;
+  const int key = static_cast<int>(keys[s]);
+  const int prev = s == 0 ? -1 : static_cast<int>(keys[s - 1]);
+  for (int k = prev + 1; k <= key; ++k) toff[k] = s;
+  if (s == n - 1)
+    for (int k = key + 1; k <= n_tiles_total; ++k) toff[k] = n;
+}
+
+// Work items: ceil(count / 64) per tile.
+__global__ void k_item_count(const int32_t* __restrict__ toff, int n_tiles,
+                             int32_t* __restrict__ icount) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tiles) return;
+  icount[k] = (toff[k + 1] - toff[k] + kWave - 1) / kWave;
+}
+
+__global__ void k_item_fill(const int32_t* __restrict__ toff, const int32_t* __restrict__ ioff,
+                            int n_tiles, const EventDesc* __restrict__ events, int n_events,
+                            Item* __restrict__ items) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_tiles) return;
+  // Event of tile k: last event with t_base <= k.
+  int lo = 0, hi = n_events - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (events[mid].t_base <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int b = toff[k], e = toff[k + 1];
+  int o = ioff[k];
+  for (int s = b; s < e; s += kWave) {
+    Item it;
+    it.tile = k;
+    it.begin = s;
+    it.end = min(e, s + kWave);
+    it.event = lo;
+    items[o++] = it;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Distances (DistanceMetrics.hpp), right fold over dimensions
+// (internal/meta/accumulate.hpp): t0 + (t1 + (... + (t_{D-1} + 0))).
+// For square-root metrics the value returned is the sum under the root.
+// ---------------------------------------------------------------------------
+template <int D, int M>
+__device__ __forceinline__ float metric_value(const float (&xi)[D], const float (&si)[D],
+                                              const float* __restrict__ sx, int64_t cs, int j,
+                                              const float* __restrict__ ssig,
+                                              const RunParams& rp) {
+  if constexpr (M == M_EUCLID) {
+    float s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float dx = xi[d] - sx[d * cs + j];
+      s = (d == D - 1) ? dx * dx : dx * dx + s;  // t + 0 == t for t = dx*dx >= +0
+    }
+    return s;
+  } else if constexpr (M == M_WEUCLID) {
+    float s = 0.0f;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float dx = xi[d] - sx[d * cs + j];
+      s = rp.mparam[d] * dx * dx + s;
+    }
+    return s;
+  } else if constexpr (M == M_PERIODIC) {
+    float s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float a = fabsf(xi[d] - sx[d * cs + j]);
+      const float pa = rp.mparam[d] - a;
+      const float m = (pa < a) ? pa : a;  // std::min(a, P - a)
+      s = (d == D - 1) ? m * m : m * m + s;
+    }
+    return s;
+  } else if constexpr (M == M_MANHATTAN) {
+    float s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float a = fabsf(xi[d] - sx[d * cs + j]);
+      s = (d == D - 1) ? a : a + s;
+    }
+    return s;
+  } else if constexpr (M == M_CHEBYSHEV) {
+    float m = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float a = fabsf(xi[d] - sx[d * cs + j]);
+      m = (m < a) ? a : m;  // std::max(m, a)
+    }
+    return m;
+  } else if constexpr (M == M_WCHEBYSHEV) {
+    float m = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const float a = rp.mparam[d] * fabsf(xi[d] - sx[d * cs + j]);
+      m = (m < a) ? a : m;
+    }
+    return m;
+  } else {  // M_MAHALANOBIS: (dx*dx) / (si*si + sj*sj)
+    float s = 0.0f;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const float dx = xi[d] - sx[d * cs + j];
+      const float sj = ssig[d * cs + j];
+      s = dx * dx / (si[d] * si[d] + sj * sj) + s;
+    }
+    return s;
+  }
+}
+
+template <int M>
+__device__ __forceinline__ float finish_distance(float v) {
+  if constexpr (MetricTraits<M>::kSqrt)
+    return __builtin_sqrtf(v);
+  else
+    return v;
+}
+
+// Search box of one point (ClusteringKernels.hpp:111-121, TilesView.hpp:77-87).
+template <int D>
+__device__ __forceinline__ void search_box(const Geo& g, const float (&xi)[D],
+                                           const float (&si)[D], float R, int (&lo)[D],
+                                           int (&hi)[D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float rs = R * si[d] * kSqrt2;
+    const float r = (R < rs) ? rs : R;  // std::max(R, R * sigma * sqrt(2))
+    int a = get_bin(g, xi[d] - r, d);
+    int b = get_bin(g, xi[d] + r, d);
+    if (g.wrapped[d] && a > b) b += g.nper;
+    lo[d] = a;
+    hi[d] = b;
+  }
+}
+
+// Walks the union of the wave's search boxes in lexicographic order (dim 0
+// outermost). For every tile that at least one lane's own box contains, calls
+// visit(tile_key, in_box) with in_box true on the lanes whose box contains it.
+// Each lane thus sees exactly its own boxes' tiles, in the reference's order.
+template <int D, class F>
+__device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D], int nper,
+                                           const int32_t* wrapped, int t_base, F&& visit) {
+  int ulo[D], uhi[D], b[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    ulo[d] = uniform(wave_min_i(lo[d]));
+    uhi[d] = uniform(wave_max_i(hi[d]));
+    b[d] = ulo[d];
+  }
+  if (ulo[0] > uhi[0]) return;  // no active lane
+  while (true) {
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) in = in && (lo[d] <= b[d]) && (b[d] <= hi[d]);
+    if (ballot(in) != 0ull) {
+      int t = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) t = t * nper + (wrapped[d] ? b[d] % nper : b[d]);
+      visit(uniform(t_base + t), in);
+    }
+    int d = D - 1;
+    for (; d >= 0; --d) {
+      if (b[d] < uhi[d]) {
+        ++b[d];
+        break;
+      }
+      b[d] = ulo[d];
+    }
+    if (d < 0) break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (b) Local density: rho_i = sum over visited j of ((float)(d <= dc) * K) * w_j
+// (ClusteringKernels.hpp:53-69). Candidates beyond dc add +0 and are skipped.
+// ---------------------------------------------------------------------------
+template <int D, int M, int K>
+__global__ __launch_bounds__(kBlock) void k_density(
+    const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
+    const int32_t* __restrict__ toff, const Item* __restrict__ items,
+    const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
+    const float* __restrict__ sw, const float* __restrict__ ssig, float* __restrict__ srho) {
+  const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (item >= *n_items) return;
+  const Item it = items[item];
+  const EventDesc ev = events[it.event];
+  const Geo g = event_geo(geo, rp, it.event, ev.nper);
+  const int s = it.begin + lane_id();
+  const bool active = s < it.end;
+
+  float xi[D], si[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xi[d] = active ? sx[d * cs + s] : 0.0f;
+    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+  }
+  int lo[D], hi[D];
+  search_box<D>(g, xi, si, rp.dc, lo, hi);
+  if (!active) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = INT32_MAX;
+      hi[d] = INT32_MIN;
+    }
+  }
+  const float thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
+  const float f = rp.kp0;
+  float rho = 0.0f;
+
+  walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+    const int j0 = toff[t];
+    const int j1 = toff[t + 1];
+    const float thr = in ? thr_in : -1.0f;
+    if (t == it.tile) {
+      for (int j = j0; j < j1; ++j) {
+        const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
+        if (v <= thr) {
+          float kv;
+          if (j == s) {
+            kv = 1.0f;
+          } else if constexpr (K == K_FLAT) {
+            kv = f;
+          } else if constexpr (K == K_EXP) {
+            kv = rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
+          } else {
+            const float dd = finish_distance<M>(v);
+            kv = rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
+          }
+          rho += kv * sw[j];
+        }
+      }
+    } else {
+      for (int j = j0; j < j1; ++j) {
+        const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
+        if constexpr (K == K_FLAT) {
+          const float term = f * sw[j];
+          rho += (v <= thr) ? term : 0.0f;
+        } else if (v <= thr) {
+          float kv;
+          if constexpr (K == K_EXP) {
+            kv = rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
+          } else {
+            const float dd = finish_distance<M>(v);
+            kv = rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
+          }
+          rho += kv * sw[j];
+        }
+      }
+    }
+  });
+  if (active) srho[s] = rho;
+}
+
+// ---------------------------------------------------------------------------
+// (c) Nearest-higher (ClusteringKernels.hpp:164-206, 247-289). The search box uses
+// dm; the acceptance radius is seed_dc for points at or above rhoc*u, else dm.
+// ---------------------------------------------------------------------------
+template <int D, int M>
+__global__ __launch_bounds__(kBlock) void k_nearest_higher(
+    const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
+    const int32_t* __restrict__ toff, const Item* __restrict__ items,
+    const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
+    const float* __restrict__ ssig, const float* __restrict__ su,
+    const uint32_t* __restrict__ stag, const float* __restrict__ srho,
+    int32_t* __restrict__ snh, float* __restrict__ sdelta) {
+  const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (item >= *n_items) return;
+  const Item it = items[item];
+  const EventDesc ev = events[it.event];
+  const Geo g = event_geo(geo, rp, it.event, ev.nper);
+  const int s = it.begin + lane_id();
+  const bool active = s < it.end;
+
+  float xi[D], si[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xi[d] = active ? sx[d * cs + s] : 0.0f;
+    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+  }
+  const float rho_i = active ? srho[s] : 0.0f;
+  const uint32_t tag_i = active ? stag[s] : 0u;
+  const float ui = (active && rp.has_u) ? su[s] : 1.0f;
+  const float eff_min = rp.rhoc * ui;
+  const bool use_seed = rho_i >= eff_min;
+  const float eff = use_seed ? rp.seed_dc : rp.dm;
+  const float eff_thr =
+      MetricTraits<M>::kSqrt ? (use_seed ? rp.seed_sq_max : rp.dm_sq_max) : eff;
+
+  int lo[D], hi[D];
+  search_box<D>(g, xi, si, rp.dm, lo, hi);
+  if (!active) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = INT32_MAX;
+      hi[d] = INT32_MIN;
+    }
+  }
+  float delta = FLT_MAX;
+  int nh = -1;
+  float rho_nh = 0.0f;
+  uint32_t tag_nh = 0u;
+
+  walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+    const int j0 = toff[t];
+    const int j1 = toff[t + 1];
+    const float thr = in ? eff_thr : -1.0f;
+    for (int j = j0; j < j1; ++j) {
+      const float rj = srho[j];
+      const uint32_t tj = stag[j];
+      const bool higher = (rj > rho_i) || ((rj == rho_i) && (rj > 0.0f) && (tj > tag_i));
+      const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
+      if (higher && v <= thr) {
+        const float dist = finish_distance<M>(v);
+        if ((dist < delta) ||
+            ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh))))) {
+          delta = dist;
+          nh = j;
+          rho_nh = rj;
+          tag_nh = tj;
+        }
+      }
+    }
+  });
+  if (active) {
+    snh[s] = nh;
+    sdelta[s] = delta;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (d) Back to input order; seed flags (ClusteringKernels.hpp:300-316).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_unpermute(
+    const RunParams rp, int n, const int32_t* __restrict__ perm, const float* __restrict__ srho,
+    const int32_t* __restrict__ snh, const float* __restrict__ sdelta,
+    const float* __restrict__ su, float* __restrict__ rho, int32_t* __restrict__ nh,
+    float* __restrict__ delta, int32_t* __restrict__ is_seed, int32_t* __restrict__ flag) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  const float r = srho[s];
+  const int h = snh[s];
+  const float u = rp.has_u ? su[s] : 1.0f;
+  const int seed = (h == -1) && (r >= rp.rhoc * u) ? 1 : 0;
+  rho[i] = r;
+  nh[i] = h >= 0 ? perm[h] : -1;
+  if (delta) delta[i] = sdelta[s];
+  is_seed[i] = seed;
+  flag[i] = seed;
+}
+
+// ---------------------------------------------------------------------------
+// (e) Labels: seeds take their rank in input order (the serial push_back order,
+// ClusteringKernels.hpp:310, 318-327); followers chase nearest_higher to a seed
+// or to a point without one (:329-343).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
+                                                   const int32_t* __restrict__ nh,
+                                                   const int32_t* __restrict__ is_seed,
+                                                   const int32_t* __restrict__ rank,
+                                                   int32_t* __restrict__ cluster,
+                                                   int32_t* __restrict__ seeds,
+                                                   int32_t* __restrict__ event_of_cluster) {
+  const Chunk ch = chunks[blockIdx.x];
+  for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
+    int c;
+    if (is_seed[i]) {
+      c = rank[i];
+      seeds[c] = i;
+      if (event_of_cluster) event_of_cluster[c] = ch.event;
+    } else if (nh[i] == -1) {
+      c = -1;
+    } else {
+      int cur = nh[i];
+      while (!is_seed[cur] && nh[cur] != -1) cur = nh[cur];
+      c = is_seed[cur] ? rank[cur] : -1;
+    }
+    cluster[i] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch over (D, metric, kernel).
+// ---------------------------------------------------------------------------
+struct NeighbourLaunch {
+  const RunParams* rp;
+  const BatchBuffers* bb;
+  int64_t cs;
+  unsigned grid;
+  hipStream_t s;
+};
+
+template <int D, int M, int K>
+static void launch_density_dmk(const NeighbourLaunch& L) {
+  const BatchBuffers& b = *L.bb;
+  hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
+                     b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
+                     b.sorted.sigma[0], b.srho);
+}
+
+template <int D, int M>
+static void launch_density_dm(const NeighbourLaunch& L) {
+  switch (L.rp->kernel) {
+    case K_FLAT: launch_density_dmk<D, M, K_FLAT>(L); break;
+    case K_EXP: launch_density_dmk<D, M, K_EXP>(L); break;
+    default: launch_density_dmk<D, M, K_GAUSS>(L); break;
+  }
+}
+
+template <int D, int M>
+static void launch_nh_dm(const NeighbourLaunch& L) {
+  const BatchBuffers& b = *L.bb;
+  hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp,
+                     b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
+                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.snh, b.sdelta);
+}
+
+template <int D>
+static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
+#define CLUE_METRIC_CASE(MID)               \
+  case MID:                                 \
+    if (density)                            \
+      launch_density_dm<D, MID>(L);         \
+    else                                    \
+      launch_nh_dm<D, MID>(L);              \
+    break;
+  switch (L.rp->metric) {
+    CLUE_METRIC_CASE(M_EUCLID)
+    CLUE_METRIC_CASE(M_WEUCLID)
+    CLUE_METRIC_CASE(M_PERIODIC)
+    CLUE_METRIC_CASE(M_MANHATTAN)
+    CLUE_METRIC_CASE(M_CHEBYSHEV)
+    CLUE_METRIC_CASE(M_WCHEBYSHEV)
+    default:
+      if (density)
+        launch_density_dm<D, M_MAHALANOBIS>(L);
+      else
+        launch_nh_dm<D, M_MAHALANOBIS>(L);
+  }
+#undef CLUE_METRIC_CASE
+}
+
+static void launch_neighbour(const NeighbourLaunch& L, bool density) {
+  switch (L.rp->D) {
+    case 1: launch_neighbour_d<1>(L, density); break;
+    case 2: launch_neighbour_d<2>(L, density); break;
+    case 3: launch_neighbour_d<3>(L, density); break;
+    case 4: launch_neighbour_d<4>(L, density); break;
+    case 5: launch_neighbour_d<5>(L, density); break;
+    case 6: launch_neighbour_d<6>(L, density); break;
+    case 7: launch_neighbour_d<7>(L, density); break;
+    case 8: launch_neighbour_d<8>(L, density); break;
+    case 9: launch_neighbour_d<9>(L, density); break;
+    default: launch_neighbour_d<10>(L, density); break;
+  }
+}
+
+const char* stage_name(int s) {
+  static const char* names[kNumStages] = {"extremes", "tiles", "density", "nearest_higher",
+                                          "seeds", "labels"};
+  return (s >= 0 && s < kNumStages) ? names[s] : "?";
+}
+
+void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuffers& bb,
+                     const ResultArrays& out, hipStream_t s, const StageHook* hook) {
+  auto mark = [&](int stage) {
+    if (hook && hook->mark) hook->mark(hook->self, stage, s);
+  };
+  const int D = rp.D;
+  const int n = bb.n_total;
+  CoordPtrs cp{};
+  for (int d = 0; d < D; ++d) cp.x[d] = in.x[d];
+
+  // (a) tiles
+  mark(kStageExtremes);
+  hipLaunchKernelGGL(k_init_extremes,
+                     dim3(static_cast<unsigned>((static_cast<int64_t>(bb.n_events) * D + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, s, bb.ext, static_cast<int64_t>(bb.n_events) * D);
+  hipLaunchKernelGGL(k_extremes, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, D, bb.chunks, bb.ext);
+  hipLaunchKernelGGL(k_geometry, dim3((bb.n_events + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                     bb.ext, bb.events, bb.n_events, D, bb.geo);
+  mark(kStageTiles);
+  hipLaunchKernelGGL(k_tile_keys, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, rp, bb.chunks,
+                     bb.events, bb.geo, bb.keys0, bb.vals0);
+  bool in_first = true;
+  radix_sort_pairs(bb.keys0, bb.vals0, bb.keys1, bb.vals1, n, bb.key_bits, bb.rs_hist, bb.scan_tmp,
+                   &in_first, s);
+  const uint32_t* skeys = in_first ? bb.keys0 : bb.keys1;
+  const int32_t* sperm = in_first ? bb.vals0 : bb.vals1;
+  GatherArgs ga;
+  ga.in = in;
+  ga.out = bb.sorted;
+  ga.cs = 0;
+  hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ga, rp, skeys,
+                     sperm, n, bb.n_tiles_total, bb.toff);
+  hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     s, bb.toff, bb.n_tiles_total, bb.icount);
+  scan_exclusive(bb.icount, bb.icount, bb.n_tiles_total, bb.scan_tmp, bb.n_items, s);
+  hipLaunchKernelGGL(k_item_fill, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                     s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items);
+
+  // (b), (c)
+  NeighbourLaunch L;
+  L.rp = &rp;
+  L.bb = &bb;
+  L.cs = bb.sorted_stride;
+  L.grid = static_cast<unsigned>((bb.max_items + kWaves - 1) / kWaves);
+  L.s = s;
+  mark(kStageDensity);
+  launch_neighbour(L, true);
+  mark(kStageNearestHigher);
+  launch_neighbour(L, false);
+
+  // (d), (e)
+  mark(kStageSeeds);
+  hipLaunchKernelGGL(k_unpermute, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
+                     bb.sorted.perm, bb.srho, bb.snh, bb.sdelta, bb.sorted.u, out.rho, out.nh,
+                     out.delta, out.is_seed, bb.flag);
+  scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
+  mark(kStageLabels);
+  hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, out.nh,
+                     out.is_seed, bb.rank, out.cluster, bb.seeds, out.event_of_cluster);
+  mark(kNumStages);
+}
+
+}  // namespace clue_dev

@@ -1,0 +1,60 @@
+// Plain structs shared by the host orchestration and the device stages.
+#pragma once
+
+#include <stdint.h>
+
+namespace clue_dev {
+
+constexpr int kMaxDim = 10;
+
+// One event of a batch. All offsets are host-computed from the event sizes.
+struct EventDesc {
+  int32_t p_off;   // first point of the event in the packed arrays
+  int32_t n;       // points in the event
+  int32_t nper;    // tiles per dimension (SetupTiles.hpp:30-34)
+  int32_t ntiles;  // nper^D
+  int32_t t_base;  // first global tile key of the event
+  int32_t pad[3];
+};
+
+// A contiguous range of points of one event, the unit of the per-point grids.
+struct Chunk {
+  int32_t event;
+  int32_t begin;  // global point index
+  int32_t end;
+  int32_t pad;
+};
+
+// A work item of the neighbour stages: up to 64 consecutive tile-sorted points of
+// one tile (one lane each).
+struct Item {
+  int32_t tile;   // global tile key
+  int32_t begin;  // sorted position of the first point
+  int32_t end;
+  int32_t event;
+};
+
+// Per-run scalar parameters as the device stages consume them.
+struct RunParams {
+  int32_t D;
+  int32_t kernel;
+  int32_t metric;
+  int32_t has_sigma;
+  int32_t has_u;
+  int32_t has_tags;
+  int32_t n_events;
+  int32_t n_total;
+  float dc;
+  float rhoc;
+  float dm;
+  float seed_dc;
+  float dc_sq_max;      // largest float t with sqrt_rn(t) <= dc (sqrt metrics)
+  float dm_sq_max;      // same for dm
+  float seed_sq_max;    // same for seed_dc
+  float kp0, kp1;       // kernel parameters (flat f | exp lambda, A | gauss sigma, A)
+  float gauss_den;      // (2 * sigma) * sigma, as the reference evaluates it
+  int32_t wrapped[kMaxDim];
+  float mparam[kMaxDim];
+};
+
+}  // namespace clue_dev

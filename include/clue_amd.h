@@ -1,0 +1,187 @@
+/*
+ * clue_amd.h — C ABI of the MI355X-native CLUE library (libclue_amd.so).
+ *
+ * Plain pointers and sizes only. Every entry point returns a clue_status; on
+ * failure clue_last_error() gives a message (thread-local). No exception
+ * crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces (file:line into
+ * the CLUEstering sources).
+ *
+ * Memory: unless stated otherwise, point and result arrays are DEVICE pointers
+ * owned by the caller. A context owns its stream, pooled workspaces and the
+ * captured graphs; like the reference's Clusterer (core/Clusterer.hpp:37-51)
+ * it is not thread-safe, and separate contexts may run concurrently.
+ */
+#ifndef CLUE_AMD_H
+#define CLUE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t clue_status;
+enum {
+  CLUE_OK = 0,
+  CLUE_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  CLUE_ERR_OUT_OF_RANGE = 2,     /* std::out_of_range (bad dimension) */
+  CLUE_ERR_RUNTIME = 3,          /* std::runtime_error (e.g. seeds before a run) */
+  CLUE_ERR_HIP = 4,              /* a HIP runtime call failed */
+  CLUE_ERR_CAPACITY = 5,         /* request exceeds a configured capacity/budget */
+  CLUE_ERR_NO_DEVICE = 6
+};
+
+/* Convolutional kernels (core/ConvolutionalKernel.hpp:19-111). */
+enum { CLUE_KERNEL_FLAT = 0, CLUE_KERNEL_EXP = 1, CLUE_KERNEL_GAUSS = 2 };
+
+/* Distance metrics (core/DistanceMetrics.hpp:64-318). */
+enum {
+  CLUE_METRIC_EUCLIDEAN = 0,
+  CLUE_METRIC_WEIGHTED_EUCLIDEAN = 1,
+  CLUE_METRIC_PERIODIC_EUCLIDEAN = 2,
+  CLUE_METRIC_MANHATTAN = 3,
+  CLUE_METRIC_CHEBYSHEV = 4,
+  CLUE_METRIC_WEIGHTED_CHEBYSHEV = 5,
+  CLUE_METRIC_MAHALANOBIS = 6
+};
+
+/* Clustering parameters: Clusterer(dc, rhoc, dm, seed_dc)
+ * (core/detail/Clusterer.hpp:33-49), setWrappedCoordinates (core/Clusterer.hpp:291-301),
+ * the kernel and the metric template arguments. */
+typedef struct {
+  int32_t ndim;            /* 1..10 */
+  int32_t kernel;          /* CLUE_KERNEL_* */
+  int32_t metric;          /* CLUE_METRIC_* */
+  int32_t points_per_tile; /* 0 selects the reference's 128 (core/Clusterer.hpp:57) */
+  int32_t wrapped[10];     /* 1 = periodic coordinate */
+  float dc;                /* density radius, > 0 */
+  float rhoc;              /* min density, >= 0 */
+  float dm;                /* outlier distance, > 0 (reference default: dc) */
+  float seed_dc;           /* seeding distance, > 0 (reference default: dc) */
+  float kparam[2];         /* flat {f}; exp {lambda, A}; gauss {sigma, A} */
+  float mparam[10];        /* per-dimension weights or periods */
+} clue_params;
+
+/* Input points, SoA columns like PointsView (data_structures/internal/PointsCommon.hpp:97-110). */
+typedef struct {
+  int32_t n;
+  const float* coords[10];          /* ndim device columns of n floats */
+  const float* weight;              /* n */
+  const float* sigma[10];           /* optional per-dimension sigma (NULL = unset) */
+  const float* density_uncertainty; /* optional (NULL = 1) */
+  const uint32_t* tags;             /* optional (NULL = point index) */
+} clue_points;
+
+/* Results, like PointsView's rho/nearest_higher/is_seed/cluster_index
+ * (PointsCommon.hpp:103-106). cluster_index is required, the rest may be NULL.
+ * delta (distance to the nearest higher, FLT_MAX when none) is not stored by the
+ * reference; it is offered for parity checks. */
+typedef struct {
+  int32_t* cluster_index;
+  int32_t* is_seed;
+  float* rho;
+  int32_t* nearest_higher;
+  float* delta;
+} clue_results;
+
+typedef struct clue_ctx clue_ctx;
+
+/* Library and device queries. listDevices (CLUEstering/BindingModules/Run.hpp:53-65). */
+const char* clue_last_error(void);
+const char* clue_version(void);
+clue_status clue_device_count(int32_t* count);
+clue_status clue_device_name(int32_t device, char* buf, int32_t buflen);
+
+/* Context: replaces the Clusterer's device state (m_tiles, m_seeds, core/Clusterer.hpp:49-51)
+ * and the alpaka Queue (utils/get_queue.hpp:19-48). stream == NULL creates an owned
+ * stream; otherwise the given hipStream_t is used (not owned). */
+clue_status clue_context_create(int32_t device, void* stream, clue_ctx** out);
+clue_status clue_context_destroy(clue_ctx* ctx);
+clue_status clue_context_stream(clue_ctx* ctx, void** stream);
+clue_status clue_synchronize(clue_ctx* ctx);
+
+/* Parameter validation as Clusterer's constructor/setParameters
+ * (core/detail/Clusterer.hpp:42-48, 80-88) and the kernel constructors
+ * (core/detail/ConvolutionalKernel.hpp:14-18, 28-33, 45-50). */
+clue_status clue_validate_params(const clue_params* params);
+
+/* Per-event, device-resident: make_clusters(queue, d_points, metric, kernel)
+ * (core/detail/Clusterer.hpp:143-151). Runs on the context stream and waits for
+ * completion, as the reference does. */
+clue_status clue_make_clusters(clue_ctx* ctx, const clue_params* params,
+                               const clue_points* points, clue_results* results);
+
+/* Same, without the final wait (the caller synchronises the stream). */
+clue_status clue_make_clusters_async(clue_ctx* ctx, const clue_params* params,
+                                     const clue_points* points, clue_results* results);
+
+/* Batched, device-resident: make_clusters(queue, d_points, batch_item_sizes, ...)
+ * (core/detail/Clusterer.hpp:171-183, 286-350). Events are packed back to back in
+ * `points`; sizes (HOST array) gives the size of each. Each event is tiled as if
+ * it were alone. Cluster ids are global and event-major, nearest_higher holds
+ * indices into the packed arrays. event_of_cluster (device, NULL allowed, capacity
+ * >= number of clusters) receives the event of each cluster
+ * (getSampleAssociations, core/detail/Clusterer.hpp:221-241). n_clusters (host,
+ * NULL allowed) receives the total number of clusters; passing it waits. */
+clue_status clue_make_clusters_batched(clue_ctx* ctx, const clue_params* params,
+                                       const clue_points* points, const uint32_t* sizes,
+                                       int32_t n_events, clue_results* results,
+                                       int32_t* event_of_cluster, int64_t* n_clusters);
+
+/* Host in, host out: make_clusters(queue, h_points, d_points[, sizes])
+ * (core/detail/Clusterer.hpp:127-138, 157-168) and the Python binding's mainRun
+ * (CLUEstering/BindingModules/Run.hpp:25-49, 69-142). host_data is the (ndim+1) x n
+ * C-contiguous SoA (coordinates then weights); host_cluster_index receives the
+ * labels; sizes (host, NULL = one event) selects the batched form. Optional host
+ * outputs may be NULL. */
+clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, int32_t n,
+                                    const float* host_data, int32_t* host_cluster_index,
+                                    const uint32_t* sizes, int32_t n_events,
+                                    int32_t* host_is_seed, float* host_rho,
+                                    int32_t* host_nearest_higher);
+
+/* Seeds of the last run, ascending point index (getSeeds, core/detail/Clusterer.hpp:198-203).
+ * `seeds` is a HOST array of capacity `capacity`; *count receives the number of seeds.
+ * CLUE_ERR_RUNTIME before the first run. */
+clue_status clue_get_seeds(clue_ctx* ctx, int32_t* seeds, int64_t capacity, int64_t* count);
+
+/* Number of clusters of the last run (n_clusters = max label + 1,
+ * utils/detail/get_cluster_properties.hpp:10-15). Waits. */
+clue_status clue_get_n_clusters(clue_ctx* ctx, int64_t* n_clusters);
+
+/* Per-stage device time of the last run in microseconds (hipEvents on the context
+ * stream); enable with clue_set_profiling. names[i] is a static string. */
+clue_status clue_set_profiling(clue_ctx* ctx, int32_t enabled);
+clue_status clue_stage_times(clue_ctx* ctx, const char** names, float* us, int32_t capacity,
+                             int32_t* count);
+
+/* Graph capture of the batched pipeline: records every stage of one batch
+ * (fixed pointers and sizes) in a hipGraph and replays it with one launch. */
+typedef struct clue_plan clue_plan;
+clue_status clue_plan_create(clue_ctx* ctx, const clue_params* params, const clue_points* points,
+                             const uint32_t* sizes, int32_t n_events, clue_results* results,
+                             int32_t* event_of_cluster, clue_plan** out);
+clue_status clue_plan_launch(clue_plan* plan); /* async on the context stream */
+clue_status clue_plan_destroy(clue_plan* plan);
+
+/* Device memory helpers for callers without their own allocator. */
+clue_status clue_malloc(clue_ctx* ctx, size_t bytes, void** ptr);
+clue_status clue_free(clue_ctx* ctx, void* ptr);
+clue_status clue_memcpy_h2d(clue_ctx* ctx, void* dst, const void* src, size_t bytes);
+clue_status clue_memcpy_d2h(clue_ctx* ctx, void* dst, const void* src, size_t bytes);
+clue_status clue_memset_d8(clue_ctx* ctx, void* dst, int value, size_t bytes);
+
+/* Device math used by the stages, exposed so that tests can pin its bits against
+ * the host: out_sqrt = sqrtf(a), out_exp = the library's expf(a), out_div = a / b.
+ * All arrays are device arrays of n floats. No reference counterpart. */
+clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a, const float* b,
+                            float* out_sqrt, float* out_exp, float* out_div);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLUE_AMD_H */

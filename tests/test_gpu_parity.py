@@ -1,0 +1,181 @@
+"""GPU parity: every output of the HIP path (rho, nearest_higher, delta, is_seed,
+cluster_index, seeds) is compared bit for bit with the oracle on the same inputs."""
+import numpy as np
+import pytest
+
+from cluestering_amd import CLUE_Convolutional_Kernels as K
+from cluestering_amd import synthetic
+from cluestering_amd.device import Plan, debug_math
+
+from _support import Case, assert_same
+from conftest import read_points
+
+pytestmark = pytest.mark.gpu
+
+
+def _workload_case(w, name):
+    return Case(w.coords, w.weight, w.dc, w.rhoc, w.dm, kernel=w.kernel, metric=w.metric,
+                wrapped=w.wrapped, sizes=w.sizes, name=name)
+
+
+def _check(case, ctx, oracle):
+    print(f"[case] {case.name} n={case.n} ndim={case.ndim}", flush=True)
+    gpu = case.run_gpu(ctx)
+    ref = case.run_oracle(oracle)
+    assert_same(gpu, ref, case.name)
+    seeds = np.nonzero(ref.is_seed)[0]
+    np.testing.assert_array_equal(gpu["seeds"], seeds)
+    return gpu, ref
+
+
+def test_device_math_bits(gpu_ctx, oracle_mod):
+    rng = np.random.default_rng(7)
+    a = np.concatenate([
+        rng.uniform(0, 10, 20000), rng.uniform(0, 1e-3, 5000), 10.0 ** rng.uniform(-40, 38, 5000),
+        np.array([0.0, 1.0, 2.0, 2.25, 4.0, np.inf], np.float64)]).astype(np.float32)
+    b = rng.uniform(0.1, 10, a.size).astype(np.float32)
+    s, _, q = debug_math(gpu_ctx, a, b)
+    np.testing.assert_array_equal(s.view(np.uint32), np.sqrt(a).view(np.uint32))
+    np.testing.assert_array_equal(q.view(np.uint32), (a / b).view(np.uint32))
+    x = np.concatenate([rng.uniform(-110, 90, 20000), rng.uniform(-2, 0, 5000),
+                        np.array([0.0, -0.0, -87.3, 88.7, 89.0, -104.0, -103.9], np.float64)])
+    x = x.astype(np.float32)
+    _, e, _ = debug_math(gpu_ctx, x, np.ones_like(x))
+    np.testing.assert_array_equal(e.view(np.uint32), oracle_mod.expf_array(x).view(np.uint32))
+
+
+def test_blobs2d_16k(gpu_ctx, oracle_mod):
+    _check(_workload_case(synthetic.blobs2d_16k(), "blobs2d_16k"), gpu_ctx, oracle_mod)
+
+
+def test_blobs2d_uniform_weights_64k(gpu_ctx, oracle_mod):
+    _check(_workload_case(synthetic.blobs2d_1m(n=1 << 16), "blobs2d_64k"), gpu_ctx, oracle_mod)
+
+
+def test_detector3d_periodic_exp_64k(gpu_ctx, oracle_mod):
+    _check(_workload_case(synthetic.detector3d_1m(n=1 << 16), "detector3d_64k"), gpu_ctx,
+           oracle_mod)
+
+
+def test_batched_gaussian(gpu_ctx, oracle_mod):
+    w = synthetic.batch2d_4096(n_events=24, lo=256, hi=8192)
+    gpu, ref = _check(_workload_case(w, "batch24"), gpu_ctx, oracle_mod)
+    assert gpu["n_clusters"] == ref.n_seeds
+
+
+@pytest.mark.parametrize("kernel", [K.FlatKernel(0.5), K.ExponentialKernel(0.5, 1.0),
+                                    K.GaussianKernel(1.0, 1.0), K.FlatKernel(2.0)])
+@pytest.mark.parametrize("metric", [None, K.EuclideanMetric(), K.EuclideanMetric([1.0, 2.0]),
+                                    K.ManhattanMetric(), K.ChebyshevMetric(),
+                                    K.ChebyshevMetric([1.0, 2.0])])
+def test_kernels_and_metrics(gpu_ctx, oracle_mod, kernel, metric):
+    c, w = read_points("data_4096.csv")
+    w = np.random.default_rng(3).uniform(0.5, 1.5, w.size).astype(np.float32)
+    _check(Case(c, w, 1.5, 10.0, 1.5, kernel=kernel, metric=metric, name=f"{kernel}-{metric}"),
+           gpu_ctx, oracle_mod)
+
+
+@pytest.mark.parametrize("ndim", list(range(1, 11)))
+def test_dimensions(gpu_ctx, oracle_mod, ndim):
+    rng = np.random.default_rng(100 + ndim)
+    n = 3000
+    centres = rng.uniform(0, 20, size=(6, ndim))
+    lab = rng.integers(0, 6, n)
+    x = (centres[lab] + rng.normal(0, 0.6, size=(n, ndim))).T.astype(np.float32)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    _check(Case(x, w, 1.2 + 0.3 * ndim, 3.0, 2.0 + 0.3 * ndim, name=f"dim{ndim}"), gpu_ctx,
+           oracle_mod)
+
+
+def test_options_tags_uncertainty_sigma(gpu_ctx, oracle_mod):
+    c, w = read_points("data_4096.csv")
+    rng = np.random.default_rng(11)
+    n = w.size
+    tags = rng.permutation(n).astype(np.uint32)
+    u = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    sigma = rng.uniform(0.0, 1.0, size=(2, n)).astype(np.float32)
+    _check(Case(c, w, 1.5, 10.0, 1.5, tags=tags, name="tags"), gpu_ctx, oracle_mod)
+    _check(Case(c, w, 1.5, 10.0, 1.5, u=u, name="uncertainty"), gpu_ctx, oracle_mod)
+    _check(Case(c, w, 1.5, 10.0, 1.5, sigma=sigma, name="sigma"), gpu_ctx, oracle_mod)
+    _check(Case(c, w, 1.5, 10.0, 1.5, sigma=sigma + 0.5, metric=K.MahalanobisMetric(),
+                name="mahalanobis"), gpu_ctx, oracle_mod)
+    _check(Case(c, w, 1.5, 10.0, 1.5, seed_dc=3.0, name="seed_dc"), gpu_ctx, oracle_mod)
+
+
+def test_periodic_opposite_angles(gpu_ctx, oracle_mod):
+    c, w = read_points("opposite_angles.csv")
+    metric = K.PeriodicEuclideanMetric([0.0, float(2 * np.float32(np.pi))])
+    gpu, _ = _check(Case(c, w, 0.2, 5.0, 0.2, metric=metric, wrapped=(0, 1), name="periodic"),
+                    gpu_ctx, oracle_mod)
+    assert gpu["cluster_index"].max() + 1 == 1  # tests/test_clustering.cpp:237-270
+
+
+def test_tag_tie_break_known_answer(gpu_ctx, oracle_mod):
+    # tests/test_clustering.cpp:517-567
+    x = np.zeros((2, 2), np.float32)
+    w = np.ones(2, np.float32)
+    gpu, _ = _check(Case(x, w, 1.0, 0.0, 1.0, name="tie"), gpu_ctx, oracle_mod)
+    assert list(gpu["nearest_higher"]) == [1, -1] and list(gpu["is_seed"]) == [0, 1]
+    gpu, _ = _check(Case(x, w, 1.0, 0.0, 1.0, tags=np.array([100, 1], np.uint32), name="tie-tags"),
+                    gpu_ctx, oracle_mod)
+    assert list(gpu["nearest_higher"]) == [-1, 0] and list(gpu["is_seed"]) == [1, 0]
+
+
+def test_mahalanobis_known_answer(gpu_ctx, oracle_mod):
+    # tests/test_mahalanobis.cpp:10-43
+    x = np.array([[2.0, 4.0]], np.float32)
+    w = np.ones(2, np.float32)
+    gpu, _ = _check(Case(x, w, 1.0, 0.3, name="maha-none"), gpu_ctx, oracle_mod)
+    assert gpu["cluster_index"].max() + 1 == 2
+    gpu, _ = _check(Case(x, w, 1.0, 0.3, metric=K.MahalanobisMetric(),
+                         sigma=np.array([[2.0, 2.0]], np.float32), name="maha-sigma"),
+                    gpu_ctx, oracle_mod)
+    assert gpu["cluster_index"].max() + 1 == 1
+
+
+def test_edge_cases(gpu_ctx, oracle_mod):
+    _check(Case(np.array([[1.0], [2.0]], np.float32), np.ones(1, np.float32), 1.0, 0.5,
+                name="single"), gpu_ctx, oracle_mod)
+    x = np.full((2, 500), 3.0, np.float32)  # zero range in every dimension
+    _check(Case(x, np.ones(500, np.float32), 1.0, 2.0, name="degenerate"), gpu_ctx, oracle_mod)
+    c, w = read_points("data_1024.csv")
+    for ppt in (1, 7, 100000):
+        _check(Case(c, w, 1.5, 10.0, 1.5, points_per_tile=ppt, name=f"ppt{ppt}"), gpu_ctx,
+               oracle_mod)
+    _check(Case(c, w, 1.5, 1e9, 1.5, name="no-seeds"), gpu_ctx, oracle_mod)
+
+
+def test_batched_with_empty_and_tiny_events(gpu_ctx, oracle_mod):
+    c, w = read_points("data_1024.csv")
+    sizes = np.array([0, 300, 1, 0, 2, 721], np.uint32)
+    _check(Case(c, w, 1.3, 10.0, 1.3, sizes=sizes, name="ragged"), gpu_ctx, oracle_mod)
+
+
+def test_batched_reference_fixture(gpu_ctx, oracle_mod):
+    # tests/test_batched_clustering.cpp:8-65: 10 copies of data_1024 give 10x the clusters.
+    c, w = read_points("data_1024.csv")
+    cb, wb = np.tile(c, (1, 10)), np.tile(w, 10)
+    gpu, ref = _check(Case(cb, wb, 1.3, 10.0, 1.3, sizes=[1024] * 10, name="batched-fixture"),
+                      gpu_ctx, oracle_mod)
+    single = Case(c, w, 1.3, 10.0, 1.3).run_gpu(gpu_ctx)
+    assert gpu["n_clusters"] == 10 * (single["cluster_index"].max() + 1)
+
+
+def test_determinism_and_graph_replay(gpu_ctx, oracle_mod):
+    w = synthetic.batch2d_4096(n_events=12, lo=256, hi=4096)
+    case = _workload_case(w, "replay")
+    ref = case.run_oracle(oracle_mod)
+    clus = case.clusterer(gpu_ctx)
+    pts = case.points(gpu_ctx)
+    plan = Plan(clus, pts, w.sizes, case.metric, case.kernel)
+    for rep in range(3):
+        pts.cluster_index.fill_bytes(0xFF)
+        pts.rho.fill_bytes(0)
+        plan.launch()
+        gpu_ctx.synchronize()
+        out = pts.results()
+        assert_same(out, ref, f"replay{rep}")
+        # eager work between replays
+        clus.make_clusters_batched(pts, w.sizes, case.metric, case.kernel)
+        assert_same(pts.results(), ref, f"eager{rep}")
+    plan.close()
