@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""CLUE benchmark on one MI355X: events/s of the batched graph path.
+
+Workload (default): 4096 2-D events, sizes log-uniform in [256, 65536], each with
+5-50 Gaussian blobs (sigma 1) in [0,100]^2 plus 20% uniform noise, weights uniform
+in [0.5,1.5], gaussian kernel (sigma 1, amplitude 1), dc=1.5, rhoc=5, dm=2; fp32.
+
+One step = clustering all events of the batch with one hipGraph launch (inputs
+resident in HBM). Printed as one JSON line, with:
+  - the plain per-event GPU path (one make_clusters call per event) on the same
+    events, and the batched/per-event speed-up;
+  - per-stage device times (hipEvents between the stages) and the roofline of
+    the dominant stage (algorithmic bytes / average duration vs 8 TB/s);
+  - the CPU baseline: the serial oracle (a restatement of the reference's serial
+    backend) timed on a bounded sample of the same events on this host;
+  - bit-exactness of the GPU labels against that oracle sample, and of the
+    per-event path against the batched one.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "events/s + p50/p99 latency, batched vs per-event GPU CLUE, bit-exact labels"
+
+
+def _workload(name):
+    from cluestering_amd import synthetic
+    if name == "batch2d_4096":
+        return synthetic.batch2d_4096()
+    if name == "blobs2d_1m":
+        return synthetic.blobs2d_1m()
+    if name == "detector3d_1m":
+        return synthetic.detector3d_1m()
+    if name == "blobs2d_16k":
+        return synthetic.blobs2d_16k()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def _describe(w):
+    if w.name == "batch2d_4096":
+        return ("4096 2-D events, sizes log-uniform [256,65536] (total %d points), 5-50 blobs "
+                "sigma 1 in [0,100]^2 + 20%% noise, weights U[0.5,1.5], gaussian kernel "
+                "(1,1), dc=1.5 rhoc=5 dm=2, fp32" % w.n)
+    return f"{w.name}: {w.n} points, {w.ndim}-D"
+
+
+def _tiles_per_event(sizes, ndim, ppt=128):
+    out = []
+    for n in sizes:
+        nt0 = (int(n) + ppt - 1) // ppt
+        k = 1
+        while k ** ndim < nt0:
+            k += 1
+        out.append(k ** ndim)
+    return np.asarray(out, np.int64)
+
+
+def _per_event_run(ctx, clus, pts, w, offsets):
+    """Plain per-event path: one synchronous make_clusters per event (the drop-in
+    make_clusters(queue, d_points) of the reference, core/detail/Clusterer.hpp:143-151)."""
+    from cluestering_amd import _capi
+    lib = _capi.load()
+    p = clus.params(w.ndim, w.metric, w.kernel)
+    lat = []
+    labels = np.empty(w.n, np.int32)
+    scratch = ctx.empty(w.n, np.int32)
+    t0 = time.perf_counter()
+    for e, n in enumerate(w.sizes):
+        o = int(offsets[e])
+        cp = _capi.Points()
+        cp.n = int(n)
+        for d in range(w.ndim):
+            cp.coords[d] = pts.coords.address(d * w.n + o)
+        cp.weight = pts.weight.address(o)
+        r = _capi.Results()
+        r.cluster_index = scratch.address(o)
+        ts = time.perf_counter()
+        _capi.check(lib.clue_make_clusters(ctx.handle, C.byref(p), C.byref(cp), C.byref(r)))
+        lat.append(time.perf_counter() - ts)
+    total = time.perf_counter() - t0
+    labels[:] = scratch.to_host()
+    return total, np.asarray(lat), labels
+
+
+def _oracle_cfg(oracle, w):
+    kernel = {0: "flat", 1: "exp", 2: "gaus"}[w.kernel.kind]
+    if w.metric is None:
+        metric, mp = "euclidean", ()
+    else:
+        mid, mp = w.metric.resolve(w.ndim)
+        metric = {0: "euclidean", 1: "weighted_euclidean", 2: "periodic_euclidean",
+                  3: "manhattan", 4: "chebyshev", 5: "weighted_chebyshev",
+                  6: "mahalanobis"}[mid]
+    return oracle.Config(ndim=w.ndim, dc=w.dc, rhoc=w.rhoc, dm=w.dm, kernel=kernel,
+                         kparams=w.kernel.params, metric=metric, mparams=mp, wrapped=w.wrapped)
+
+
+def cpu_baseline(w, offsets, gpu_labels, seconds):
+    """Serial oracle over the events in order until `seconds` of CPU work; returns
+    events/s, the sample, and whether the GPU labels of the sample are bit-exact
+    (batched ids = per-event ids + seeds of the previous events)."""
+    from oracle import oracle
+    cfg = _oracle_cfg(oracle, w)
+    done, exact, t_cpu = 0, True, 0.0
+    first_id = 0
+    points = 0
+    for e, n in enumerate(w.sizes):
+        o, n = int(offsets[e]), int(n)
+        c = np.ascontiguousarray(w.coords[:, o:o + n])
+        t0 = time.perf_counter()
+        r = oracle.run(cfg, c, w.weight[o:o + n], omp=False)
+        t_cpu += time.perf_counter() - t0
+        want = np.where(r.cluster >= 0, r.cluster + first_id, -1)
+        exact = exact and bool(np.array_equal(gpu_labels[o:o + n], want))
+        first_id += r.n_seeds
+        done += 1
+        points += n
+        if t_cpu >= seconds:
+            break
+    return done / t_cpu, done, points, t_cpu, exact
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="batch2d_4096")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-event", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus > 1) are not "
+              "supported (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    from cluestering_amd.device import Clusterer, Context, Plan, PointsDevice
+
+    w = _workload(args.workload)
+    sizes = w.sizes if w.sizes is not None else np.array([w.n], np.uint32)
+    w.sizes = sizes
+    offsets = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))])[:-1]
+    n_events = len(sizes)
+    print(f"[bench] workload {w.name}: {n_events} events, {w.n} points", file=sys.stderr,
+          flush=True)
+
+    ctx = Context(0)
+    pts = PointsDevice.from_host(ctx, w.coords, w.weight)
+    clus = Clusterer(ctx, w.dc, w.rhoc, w.dm)
+    clus.set_wrapped_coordinates(w.wrapped)
+    plan = Plan(clus, pts, sizes, w.metric, w.kernel)
+
+    for _ in range(args.warmup):
+        plan.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.launch()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n_events / (ms_per_step / 1e3)
+    print(f"[bench] batched graph: {ms_per_step:.3f} ms/step, {value:.1f} events/s",
+          file=sys.stderr, flush=True)
+    batched_labels = pts.cluster_index.to_host()
+
+    # Per-stage device times: hipEvents between the stages on the context stream, on
+    # the same kernels run eagerly (same batch, same launches as the graph).
+    acc = {}
+    reps = max(3, min(args.steps, 10))
+    ctx.set_profiling(True)
+    for _ in range(reps):
+        clus.make_clusters_batched(pts, sizes, w.metric, w.kernel)
+        for k, v in ctx.stage_times().items():
+            acc[k] = acc.get(k, 0.0) + v / reps
+    ctx.set_profiling(False)
+    stage_us = {k: round(v, 2) for k, v in acc.items()}
+
+    # Roofline of the dominant neighbour stage (algorithmic bytes per launch).
+    tiles = _tiles_per_event(sizes, w.ndim)
+    D = w.ndim
+    alg = {
+        "density": w.n * (4 * (D + 1) + 4) + 4 * int((tiles + 1).sum()),
+        "nearest_higher": w.n * (4 * (D + 1) + 8) + 4 * int((tiles + 1).sum()),
+    }
+    dom = max(alg, key=lambda k: acc.get(k, 0.0))
+    dom_us = acc[dom]
+    achieved = alg[dom] / (dom_us * 1e-6) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{w.name}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+                "kernel": f"k_{dom}", "alg_bytes_per_launch": alg[dom],
+                "avg_launch_us": round(dom_us, 2)}
+    pipeline_bytes = w.n * (88 if D == 2 else 104)
+    pipeline_us = sum(acc.values())
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "events/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": _describe(w), "events": int(n_events), "points": int(w.n),
+                   "path": "batched, one hipGraph launch per step"},
+        "roofline": roofline,
+        "stage_us": stage_us,
+        "pipeline_hbm_frac": round(pipeline_bytes / (pipeline_us * 1e-6) / 1e9 / PEAK_HBM_GBS, 5),
+        "batched_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
+    }
+
+    if not args.no_per_event:
+        total, lat, labels = _per_event_run(ctx, clus, pts, w, offsets)
+        # Per-event ids are local; the batched ids add the seeds of previous events.
+        first = 0
+        same = True
+        for e, n in enumerate(sizes):
+            o, n = int(offsets[e]), int(n)
+            loc = labels[o:o + n]
+            same = same and bool(np.array_equal(batched_labels[o:o + n],
+                                                np.where(loc >= 0, loc + first, -1)))
+            first += int(loc.max()) + 1 if n and loc.max() >= 0 else 0
+        pe = n_events / total
+        out["per_event"] = {"events_per_s": round(pe, 2),
+                            "latency_us": {"p50": round(float(np.percentile(lat, 50)) * 1e6, 1),
+                                           "p99": round(float(np.percentile(lat, 99)) * 1e6, 1)},
+                            "labels_equal_batched": same}
+        out["speedup_batched_vs_per_event"] = round(value / pe, 2)
+        print(f"[bench] per-event path: {pe:.1f} events/s", file=sys.stderr, flush=True)
+
+    if not args.no_cpu:
+        from oracle import oracle
+        eps, done, npts, secs, exact = cpu_baseline(w, offsets, batched_labels, args.cpu_seconds)
+        out["cpu_baseline"] = {
+            "value": round(eps, 3), "unit": "events/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} events ({npts} points) of the same batch, serial oracle "
+                      f"(restatement of the reference's serial backend), {secs:.1f} s",
+            "omp_threads_available": oracle.num_threads(True)}
+        out["labels_bit_exact"] = {"vs_oracle_sample_events": done, "equal": exact}
+        print(f"[bench] cpu baseline: {eps:.2f} events/s over {done} events", file=sys.stderr,
+              flush=True)
+
+    plan.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,298 @@
+"""The Python ``clusterer`` of the reference (CLUEstering/CLUEstering.py:214-827)
+on the MI355X backend: input parsing into the (Ndim+1) x N float SoA, kernel and
+metric choice, ``run_clue`` / ``fit`` / ``fit_predict`` and the result accessors.
+Plotting is not part of this package.
+
+Backends: only "gpu hip" is built here; it is the default.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from typing import Optional, Union
+
+import numpy as np
+
+from . import CLUE_Convolutional_Kernels as clue_kernels
+from . import CLUE_GPU_HIP as gpu_hip
+
+EuclideanMetric = clue_kernels.EuclideanMetric
+ManhattanMetric = clue_kernels.ManhattanMetric
+ChebyshevMetric = clue_kernels.ChebyshevMetric
+PeriodicEuclideanMetric = clue_kernels.PeriodicEuclideanMetric
+
+backends = ["gpu hip"]
+
+
+def is_hip_available() -> bool:
+    from .device import device_count
+    return device_count() > 0
+
+
+@dataclass()
+class ClusteringDataSoA:
+    """Input coordinates (with the weights as last row) and the cluster ids."""
+
+    coords: np.ndarray = None
+    results: np.ndarray = None
+    n_dim: int = None
+    n_points: int = None
+
+
+@dataclass(eq=False)
+class cluster_properties:
+    """Results of a run (CLUEstering.py:181-211); equality compares n_clusters."""
+
+    n_clusters: int
+    cluster_ids: np.ndarray
+    cluster_points: np.ndarray
+    points_per_cluster: np.ndarray
+    output_df: object
+
+    def __eq__(self, other):
+        return self.n_clusters == other.n_clusters
+
+
+class clusterer:
+    """CLUE clusterer (CLUEstering.py:214-827).
+
+    :param density_radius: radius of the local density (dc)
+    :param min_density: density threshold separating seeds from outliers (rhoc)
+    :param outlier_distance: follower search radius (dm), defaults to density_radius
+    :param seeding_distance: seed search radius, defaults to density_radius
+    """
+
+    def __init__(self, density_radius: float, min_density: float,
+                 outlier_distance: Optional[float] = None,
+                 seeding_distance: Optional[float] = None):
+        self._density_radius = density_radius
+        self._min_density = min_density
+        self._outlier_distance = density_radius if outlier_distance is None else outlier_distance
+        self._seeding_distance = density_radius if seeding_distance is None else seeding_distance
+        self.clust_data: Optional[ClusteringDataSoA] = None
+        self._kernel = clue_kernels.FlatKernel(0.5)
+        self._metric = clue_kernels.EuclideanMetric()
+        self.wrapped = None
+        self.clust_prop: Optional[cluster_properties] = None
+        self._elapsed_time = 0.0
+
+    def set_params(self, density_radius: float, min_density: float,
+                   outlier_distance: Optional[float] = None,
+                   seeding_distance: Optional[float] = None) -> None:
+        self._density_radius = density_radius
+        self._min_density = min_density
+        self._outlier_distance = density_radius if outlier_distance is None else outlier_distance
+        self._seeding_distance = density_radius if seeding_distance is None else seeding_distance
+
+    # -- input (CLUEstering.py:291-404) -------------------------------------------------
+    def _read_array(self, input_data) -> None:
+        if len(input_data) < 2 or len(input_data) > 11:
+            raise ValueError("Inadequate data. The supported dimensions are between" +
+                             "1 and 10.")
+        npoints = len(input_data[-1])
+        ndim = len(input_data[:-1])
+        dtype = np.asarray(input_data[0][0]).dtype
+        if not np.issubdtype(dtype, np.floating):
+            dtype = np.float64
+        coords = np.vstack([np.asarray(input_data[:-1], dtype=dtype),
+                            np.asarray(input_data[-1], dtype=dtype)])
+        coords = np.ascontiguousarray(coords, dtype=dtype)
+        self.clust_data = ClusteringDataSoA(coords, np.zeros(npoints, dtype=np.int32), ndim,
+                                            npoints)
+
+    def _read_string(self, input_data: str, dtype=np.float32):
+        import pandas as pd
+        if not input_data.endswith(".csv"):
+            raise ValueError("Wrong type of file. The file is not a csv file.")
+        return pd.read_csv(input_data, dtype=dtype)
+
+    def _read_dict_df(self, input_data, dtype=np.float32):
+        import pandas as pd
+        return pd.DataFrame(input_data, copy=False, dtype=dtype)
+
+    def _handle_dataframe(self, df_) -> None:
+        if len(df_.columns) < 2:
+            raise ValueError("Inadequate data. The data must contain at least one coordinate "
+                             "and the weight.")
+        if len(df_.columns) > 11:
+            raise ValueError("Inadequate data. The maximum number of dimensions supported is 10.")
+        ndim = len(df_.columns) - 1
+        npoints = len(df_.index)
+        coords = df_.iloc[:, 0:-1].to_numpy()
+        coords = np.vstack([coords.T, df_.iloc[:, -1]]).astype(df_.dtypes.iloc[0])
+        coords = np.ascontiguousarray(coords, dtype=df_.dtypes.iloc[0])
+        self.clust_data = ClusteringDataSoA(coords, np.zeros(npoints, dtype=np.int32), ndim,
+                                            npoints)
+
+    def read_data(self, input_data, wrapped_coords=None) -> None:
+        """Reads a DataFrame, a CSV path, a dict, or a list/array of coordinate rows
+        followed by the weight row."""
+        import pandas as pd
+        if isinstance(input_data, (list, np.ndarray)):
+            self._read_array(input_data)
+        if isinstance(input_data, str):
+            self._handle_dataframe(self._read_string(input_data))
+        if isinstance(input_data, (dict, pd.DataFrame)):
+            self._handle_dataframe(self._read_dict_df(input_data))
+        if self.clust_data is None:
+            raise ValueError("unsupported input data type")
+        self.wrapped = list(wrapped_coords) if wrapped_coords is not None else \
+            [0] * self.clust_data.n_dim
+
+    def set_wrapped(self, wrapped_coords) -> None:
+        self.wrapped = list(wrapped_coords)
+
+    # -- kernel and metric (CLUEstering.py:418-496) -------------------------------------
+    def choose_kernel(self, choice: str, parameters=None) -> None:
+        if choice == "flat":
+            if len(parameters) != 1:
+                raise ValueError("Wrong number of parameters. The flat kernel requires 1 parameter.")
+            self._kernel = clue_kernels.FlatKernel(parameters[0])
+        elif choice == "exp":
+            if len(parameters) != 2:
+                raise ValueError("Wrong number of parameters. The exponential kernel requires 2 "
+                                 "parameters.")
+            self._kernel = clue_kernels.ExponentialKernel(parameters[0], parameters[1])
+        elif choice == "gaus":
+            if len(parameters) != 2:
+                raise ValueError("Wrong number of parameters. The gaussian kernel requires 2 "
+                                 "parameters.")
+            self._kernel = clue_kernels.GaussianKernel(parameters[0], parameters[1])
+        else:
+            raise ValueError("Invalid kernel. Allowed choices are: flat, exp, gaus.")
+
+    def choose_metric(self, choice: str, parameters=None) -> None:
+        if choice == "euclidean":
+            self._metric = clue_kernels.EuclideanMetric()
+        elif choice == "manhattan":
+            self._metric = clue_kernels.ManhattanMetric()
+        elif choice == "chebyshev":
+            self._metric = clue_kernels.ChebyshevMetric()
+        elif choice == "weighted_euclidean":
+            if not parameters:
+                raise ValueError("weighted_euclidean requires a 'parameters' list of "
+                                 "per-dimension weights.")
+            self._metric = clue_kernels.EuclideanMetric(list(parameters))
+        elif choice == "weighted_chebyshev":
+            if not parameters:
+                raise ValueError("weighted_chebyshev requires a 'parameters' list of "
+                                 "per-dimension weights.")
+            self._metric = clue_kernels.ChebyshevMetric(list(parameters))
+        elif choice == "periodic_euclidean":
+            if not parameters:
+                raise ValueError("periodic_euclidean requires a 'parameters' list of "
+                                 "per-dimension periods.")
+            self._metric = clue_kernels.PeriodicEuclideanMetric(list(parameters))
+        else:
+            raise ValueError("Invalid metric. Allowed choices are: euclidean, manhattan, "
+                             "chebyshev, weighted_euclidean, weighted_chebyshev, "
+                             "periodic_euclidean.")
+
+    # -- accessors -------------------------------------------------------------------------
+    @property
+    def coords(self) -> np.ndarray:
+        return self.clust_data.coords[:-1]
+
+    @property
+    def weight(self) -> np.ndarray:
+        return self.clust_data.coords[-1]
+
+    @property
+    def n_dim(self) -> int:
+        return self.clust_data.n_dim
+
+    @property
+    def n_points(self) -> int:
+        return self.clust_data.n_points
+
+    def list_devices(self, backend: str = "all") -> None:
+        if backend in ("all", "gpu hip"):
+            gpu_hip.listDevices("gpu hip")
+        else:
+            raise ValueError(f"Invalid backend. This package provides: {', '.join(backends)}.")
+
+    def _partial_dimension_dataset(self, dimensions) -> np.ndarray:
+        coords = [np.copy(self.clust_data.coords[dim]) for dim in dimensions]
+        coords.append(np.copy(self.clust_data.coords[-1]))
+        return np.ascontiguousarray(coords, dtype=self.clust_data.coords.dtype)
+
+    # -- run (CLUEstering.py:609-690) ------------------------------------------------------
+    def run_clue(self, backend: str = "gpu hip", batch_sample_sizes=None, device_id: int = 0,
+                 verbose: bool = False, dimensions=None) -> None:
+        if backend != "gpu hip":
+            raise ValueError(f"Invalid backend '{backend}'. This package provides: "
+                             f"{', '.join(backends)}.")
+        if dimensions is None:
+            data = self.clust_data
+        else:
+            data = ClusteringDataSoA(self._partial_dimension_dataset(dimensions),
+                                     np.copy(self.clust_data.results), len(dimensions),
+                                     self.clust_data.n_points)
+        wrapped = self.wrapped if self.wrapped is not None else [0] * data.n_dim
+        if dimensions is not None:
+            wrapped = [wrapped[d] for d in dimensions]
+        start = time.time_ns()
+        gpu_hip.mainRun(self._density_radius, self._min_density, self._outlier_distance,
+                        self._seeding_distance, wrapped, data.coords, data.results, self._kernel,
+                        data.n_dim, batch_sample_sizes, data.n_points, device_id, self._metric)
+        finish = time.time_ns()
+        cluster_ids = data.results
+        n_clusters = int(np.max(cluster_ids)) + 1 if cluster_ids.size else 0
+        valid = np.nonzero(cluster_ids >= 0)[0]
+        order = valid[np.argsort(cluster_ids[valid], kind="stable")]
+        counts = np.bincount(cluster_ids[valid], minlength=max(n_clusters, 0))
+        splits = np.split(order, np.cumsum(counts)[:-1]) if n_clusters > 0 else []
+        cluster_points = np.empty(len(splits), dtype=object)
+        for k, s in enumerate(splits):
+            cluster_points[k] = list(s)
+        try:
+            import pandas as pd
+            output_df = pd.DataFrame({"cluster_ids": cluster_ids})
+        except ImportError:  # pragma: no cover
+            output_df = None
+        self.clust_prop = cluster_properties(n_clusters, cluster_ids, cluster_points,
+                                             np.asarray(counts), output_df)
+        self._elapsed_time = (finish - start) / 1e6
+        if verbose:
+            print(f"CLUE executed in {self._elapsed_time} ms")
+            print(f"Number of clusters found: {self.clust_prop.n_clusters}")
+
+    def fit(self, data, backend: str = "gpu hip", batch_sample_sizes=None, device_id: int = 0,
+            verbose: bool = False, dimensions=None) -> "clusterer":
+        self.read_data(data)
+        self.run_clue(backend, batch_sample_sizes, device_id, verbose, dimensions)
+        return self
+
+    def fit_predict(self, data, backend: str = "gpu hip", batch_sample_sizes=None,
+                    device_id: int = 0, verbose: bool = False, dimensions=None) -> np.ndarray:
+        self.read_data(data)
+        self.run_clue(backend, batch_sample_sizes, device_id, verbose, dimensions)
+        return self.cluster_ids
+
+    @property
+    def n_clusters(self) -> int:
+        return self.clust_prop.n_clusters
+
+    @property
+    def cluster_ids(self) -> np.ndarray:
+        return self.clust_prop.cluster_ids
+
+    @property
+    def labels(self) -> np.ndarray:
+        return self.clust_prop.cluster_ids
+
+    @property
+    def cluster_points(self) -> np.ndarray:
+        return self.clust_prop.cluster_points
+
+    @property
+    def points_per_cluster(self) -> np.ndarray:
+        return self.clust_prop.points_per_cluster
+
+    @property
+    def output_df(self):
+        return self.clust_prop.output_df
+
+    @property
+    def elapsed_time(self) -> float:
+        return self._elapsed_time

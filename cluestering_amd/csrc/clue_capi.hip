@@ -275,6 +275,21 @@ void mark_stage(void* self, int stage, hipStream_t s) {
   if (hipEventRecord(c->ev[stage], s) == hipSuccess) c->ev_recorded[stage] = true;
 }
 
+clue_status read_stage_times(hipEvent_t* ev, const bool* recorded, const char** names, float* us,
+                             int32_t capacity, int32_t* count) {
+  int k = 0;
+  for (int s = 0; s < kNumStages && k < capacity; ++s) {
+    if (!recorded[s] || !recorded[s + 1]) continue;
+    float ms = 0.0f;
+    CLUE_HIP(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+    if (names) names[k] = stage_name(s);
+    if (us) us[k] = ms * 1000.0f;
+    ++k;
+  }
+  *count = k;
+  return CLUE_OK;
+}
+
 // Reserves the workspace of one batch and fills the BatchBuffers. Descriptor
 // uploads are enqueued on the stream (they precede any kernel that reads them).
 clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params* p,
@@ -617,17 +632,7 @@ clue_status clue_stage_times(clue_ctx* ctx, const char** names, float* us, int32
   *count = 0;
   if (!ctx->ev_ready) return fail(CLUE_ERR_RUNTIME, "profiling was not enabled for the last run");
   CLUE_HIP(hipStreamSynchronize(ctx->stream));
-  int k = 0;
-  for (int s = 0; s < kNumStages && k < capacity; ++s) {
-    if (!ctx->ev_recorded[s] || !ctx->ev_recorded[s + 1]) continue;
-    float ms = 0.0f;
-    CLUE_HIP(hipEventElapsedTime(&ms, ctx->ev[s], ctx->ev[s + 1]));
-    if (names) names[k] = stage_name(s);
-    if (us) us[k] = ms * 1000.0f;
-    ++k;
-  }
-  *count = k;
-  return CLUE_OK;
+  return read_stage_times(ctx->ev, ctx->ev_recorded, names, us, capacity, count);
 }
 
 clue_status clue_plan_create(clue_ctx* ctx, const clue_params* params, const clue_points* points,

@@ -1,0 +1,689 @@
+// Header-only C++ drop-in for the reference's clue::Clusterer / PointsHost /
+// PointsDevice API (include/CLUEstering/core/Clusterer.hpp:37-312,
+// data_structures/PointsHost.hpp:29-257, data_structures/PointsDevice.hpp:32),
+// running on libclue_amd.so through the C ABI in clue_amd.h.
+//
+// Link with -lclue_amd. Single precision (the library's device path is fp32).
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <fstream>
+#include <memory>
+#include <optional>
+#include <ranges>
+#include <span>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../clue_amd.h"
+
+namespace clue {
+
+namespace detail {
+
+inline void check(clue_status s) {
+  if (s == CLUE_OK) return;
+  const std::string msg = clue_last_error();
+  switch (s) {
+    case CLUE_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+    case CLUE_ERR_OUT_OF_RANGE: throw std::out_of_range(msg);
+    default: throw std::runtime_error(msg);
+  }
+}
+
+template <class T>
+struct DeviceDeleter {
+  clue_ctx* ctx;
+  void operator()(T* p) const {
+    if (p) clue_free(ctx, p);
+  }
+};
+
+}  // namespace detail
+
+namespace concepts {
+template <class M>
+concept distance_metric = M::is_metric && requires(const M& m, clue_params& p) { m.fill(p); };
+template <class K>
+concept convolutional_kernel = K::is_kernel && requires(const K& k, clue_params& p) { k.fill(p); };
+}  // namespace concepts
+
+// The reference's Queue (utils/get_queue.hpp:19-48): a device and a stream.
+class Queue {
+ public:
+  explicit Queue(int device = 0, void* stream = nullptr) {
+    clue_ctx* c = nullptr;
+    detail::check(clue_context_create(device, stream, &c));
+    m_ctx = std::shared_ptr<clue_ctx>(c, [](clue_ctx* p) { clue_context_destroy(p); });
+    m_device = device;
+  }
+  clue_ctx* ctx() const { return m_ctx.get(); }
+  int device() const { return m_device; }
+  void wait() const { detail::check(clue_synchronize(m_ctx.get())); }
+  void* stream() const {
+    void* s = nullptr;
+    detail::check(clue_context_stream(m_ctx.get(), &s));
+    return s;
+  }
+
+ private:
+  std::shared_ptr<clue_ctx> m_ctx;
+  int m_device = 0;
+};
+
+inline Queue get_queue(int device_id = 0) { return Queue(device_id); }
+inline void wait(const Queue& q) { q.wait(); }
+
+// ---- Convolutional kernels (core/ConvolutionalKernel.hpp:19-111) -------------------------
+template <class T = float>
+class FlatKernel {
+ public:
+  using value_type = T;
+  static constexpr bool is_kernel = true;
+  FlatKernel(T flat) : m_flat{flat} {
+    if (!(flat > T{0})) throw std::invalid_argument("Flat kernel parameter must be positive.");
+  }
+  void fill(clue_params& p) const {
+    p.kernel = CLUE_KERNEL_FLAT;
+    p.kparam[0] = static_cast<float>(m_flat);
+  }
+
+ private:
+  T m_flat;
+};
+
+template <class T = float>
+class ExponentialKernel {
+ public:
+  using value_type = T;
+  static constexpr bool is_kernel = true;
+  ExponentialKernel(T avg, T amplitude) : m_avg{avg}, m_amp{amplitude} {
+    if (!(avg > T{0}) || !(amplitude > T{0}))
+      throw std::invalid_argument("Exponential kernel parameters must be positive.");
+  }
+  void fill(clue_params& p) const {
+    p.kernel = CLUE_KERNEL_EXP;
+    p.kparam[0] = static_cast<float>(m_avg);
+    p.kparam[1] = static_cast<float>(m_amp);
+  }
+
+ private:
+  T m_avg, m_amp;
+};
+
+template <class T = float>
+class GaussianKernel {
+ public:
+  using value_type = T;
+  static constexpr bool is_kernel = true;
+  GaussianKernel(T stddev, T amplitude) : m_std{stddev}, m_amp{amplitude} {
+    if (!(stddev > T{0}) || !(amplitude > T{0}))
+      throw std::invalid_argument("Gaussian kernel parameters must be positive.");
+  }
+  void fill(clue_params& p) const {
+    p.kernel = CLUE_KERNEL_GAUSS;
+    p.kparam[0] = static_cast<float>(m_std);
+    p.kparam[1] = static_cast<float>(m_amp);
+  }
+
+ private:
+  T m_std, m_amp;
+};
+
+// ---- Distance metrics (core/DistanceMetrics.hpp:64-371) ----------------------------------
+template <std::size_t Ndim, class T = float>
+struct EuclideanMetric {
+  static constexpr bool is_metric = true;
+  void fill(clue_params& p) const { p.metric = CLUE_METRIC_EUCLIDEAN; }
+};
+template <std::size_t Ndim, class T = float>
+struct WeightedEuclideanMetric {
+  static constexpr bool is_metric = true;
+  std::array<T, Ndim> weights;
+  WeightedEuclideanMetric(const std::array<T, Ndim>& w) : weights{w} {}
+  template <class... V>
+    requires(sizeof...(V) == Ndim)
+  WeightedEuclideanMetric(V... w) : weights{static_cast<T>(w)...} {}
+  void fill(clue_params& p) const {
+    p.metric = CLUE_METRIC_WEIGHTED_EUCLIDEAN;
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(weights[d]);
+  }
+};
+template <std::size_t Ndim, class T = float>
+struct PeriodicEuclideanMetric {
+  static constexpr bool is_metric = true;
+  std::array<T, Ndim> periods;
+  PeriodicEuclideanMetric(const std::array<T, Ndim>& per) : periods{per} {}
+  template <class... V>
+    requires(sizeof...(V) == Ndim)
+  PeriodicEuclideanMetric(V... per) : periods{static_cast<T>(per)...} {}
+  void fill(clue_params& p) const {
+    p.metric = CLUE_METRIC_PERIODIC_EUCLIDEAN;
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(periods[d]);
+  }
+};
+template <std::size_t Ndim, class T = float>
+struct ManhattanMetric {
+  static constexpr bool is_metric = true;
+  void fill(clue_params& p) const { p.metric = CLUE_METRIC_MANHATTAN; }
+};
+template <std::size_t Ndim, class T = float>
+struct ChebyshevMetric {
+  static constexpr bool is_metric = true;
+  void fill(clue_params& p) const { p.metric = CLUE_METRIC_CHEBYSHEV; }
+};
+template <std::size_t Ndim, class T = float>
+struct WeightedChebyshevMetric {
+  static constexpr bool is_metric = true;
+  std::array<T, Ndim> weights;
+  WeightedChebyshevMetric(const std::array<T, Ndim>& w) : weights{w} {}
+  template <class... V>
+    requires(sizeof...(V) == Ndim)
+  WeightedChebyshevMetric(V... w) : weights{static_cast<T>(w)...} {}
+  void fill(clue_params& p) const {
+    p.metric = CLUE_METRIC_WEIGHTED_CHEBYSHEV;
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(weights[d]);
+  }
+};
+template <std::size_t Ndim, class T = float>
+struct MahalanobisMetric {
+  static constexpr bool is_metric = true;
+  void fill(clue_params& p) const { p.metric = CLUE_METRIC_MAHALANOBIS; }
+};
+
+namespace metrics {
+template <std::size_t Ndim, class T = float>
+using Euclidean = EuclideanMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using WeightedEuclidean = WeightedEuclideanMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using PeriodicEuclidean = PeriodicEuclideanMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using Manhattan = ManhattanMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using Chebyshev = ChebyshevMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using WeightedChebyshev = WeightedChebyshevMetric<Ndim, T>;
+template <std::size_t Ndim, class T = float>
+using Mahalanobis = MahalanobisMetric<Ndim, T>;
+}  // namespace metrics
+
+// ---- Host CSR of cluster -> points (AssociationMapHost, data_structures/AssociationMap.hpp) --
+class AssociationMapHost {
+ public:
+  AssociationMapHost() = default;
+  AssociationMapHost(std::vector<int32_t> offsets, std::vector<int32_t> indexes)
+      : m_off(std::move(offsets)), m_idx(std::move(indexes)) {}
+  // Builds key -> values from an association array (utils/detail/get_clusters.hpp:24-49):
+  // keys 0..max, values ascending, negative keys dropped.
+  static AssociationMapHost from_associations(std::span<const int32_t> assoc) {
+    int32_t nkeys = 0;
+    for (int32_t k : assoc) nkeys = std::max(nkeys, k + 1);
+    std::vector<int32_t> off(static_cast<std::size_t>(nkeys) + 1, 0);
+    for (int32_t k : assoc)
+      if (k >= 0) off[k + 1]++;
+    for (int32_t k = 0; k < nkeys; ++k) off[k + 1] += off[k];
+    std::vector<int32_t> fill(off.begin(), off.end() - 1), idx(off.back());
+    for (std::size_t i = 0; i < assoc.size(); ++i)
+      if (assoc[i] >= 0) idx[fill[assoc[i]]++] = static_cast<int32_t>(i);
+    return AssociationMapHost(std::move(off), std::move(idx));
+  }
+  std::size_t size() const { return m_off.empty() ? 0 : m_off.size() - 1; }
+  std::span<const int32_t> operator[](int32_t key) const {
+    return {m_idx.data() + m_off[key], static_cast<std::size_t>(m_off[key + 1] - m_off[key])};
+  }
+  std::size_t count(int32_t key) const { return static_cast<std::size_t>(m_off[key + 1] - m_off[key]); }
+  const int32_t* lower_bound(int32_t key) const { return m_idx.data() + m_off[key]; }
+  const int32_t* upper_bound(int32_t key) const { return m_idx.data() + m_off[key + 1]; }
+  const std::vector<int32_t>& offsets() const { return m_off; }
+  const std::vector<int32_t>& indexes() const { return m_idx; }
+
+ private:
+  std::vector<int32_t> m_off, m_idx;
+};
+
+// ---- Points ------------------------------------------------------------------------------
+// Host SoA (detail/HostViewPartition.hpp:15-21): Ndim coordinate columns, weights, and
+// the cluster indexes, in one owned buffer or in caller buffers.
+template <std::size_t Ndim, class T = float>
+class PointsHost {
+  static_assert(std::is_same_v<std::remove_cv_t<T>, float>,
+                "the MI355X device path is single precision");
+
+ public:
+  using value_type = float;
+
+  PointsHost(Queue&, int32_t n) : m_n{check_n(n)}, m_own((Ndim + 1) * static_cast<std::size_t>(n)),
+                                  m_own_idx(static_cast<std::size_t>(n), 0) {
+    for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = m_own.data() + d * n;
+    m_weight = m_own.data() + Ndim * static_cast<std::size_t>(n);
+    m_cluster = m_own_idx.data();
+  }
+  // (Ndim+1) x n SoA input plus an n-int output (the Python binding's layout, Run.hpp:39).
+  PointsHost(Queue&, int32_t n, float* input, int* output) : m_n{check_n(n)} {
+    for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = input + d * n;
+    m_weight = input + Ndim * static_cast<std::size_t>(n);
+    m_cluster = output;
+  }
+  PointsHost(Queue& q, int32_t n, std::span<float> input, std::span<int> output)
+      : PointsHost(q, n, input.data(), output.data()) {
+    if (input.size() < (Ndim + 1) * static_cast<std::size_t>(n) || output.size() < static_cast<std::size_t>(n))
+      throw std::invalid_argument("buffers too small for the number of points");
+  }
+  PointsHost(Queue&, int32_t n, float* coordinates, float* weights, int* output) : m_n{check_n(n)} {
+    for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = coordinates + d * n;
+    m_weight = weights;
+    m_cluster = output;
+  }
+  PointsHost(const PointsHost&) = delete;
+  PointsHost& operator=(const PointsHost&) = delete;
+  PointsHost(PointsHost&&) = default;
+  PointsHost& operator=(PointsHost&&) = default;
+
+  int32_t size() const { return m_n; }
+  std::span<float> coords(std::size_t dim) {
+    if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
+    return {m_coords[dim], static_cast<std::size_t>(m_n)};
+  }
+  std::span<const float> coords(std::size_t dim) const {
+    if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
+    return {m_coords[dim], static_cast<std::size_t>(m_n)};
+  }
+  std::span<float> weights() { return {m_weight, static_cast<std::size_t>(m_n)}; }
+  std::span<const float> weights() const { return {m_weight, static_cast<std::size_t>(m_n)}; }
+  std::span<const int> clusterIndexes() const { return {m_cluster, static_cast<std::size_t>(m_n)}; }
+  std::span<int> clusterIndexes() { return {m_cluster, static_cast<std::size_t>(m_n)}; }
+  bool clustered() const { return m_clustered; }
+  void mark_clustered() { m_clustered = true; }
+
+  std::size_t n_clusters() const {
+    int32_t m = -1;
+    for (int32_t i = 0; i < m_n; ++i) m = std::max(m, m_cluster[i]);
+    return static_cast<std::size_t>(m + 1);
+  }
+  void set_density_uncertainty(std::span<float> u) { m_u = check_span(u).data(); }
+  void set_sigma(std::size_t dim, std::span<float> s) {
+    if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to set_sigma.");
+    m_sigma[dim] = check_span(s).data();
+  }
+  void set_tags(std::span<std::uint32_t> tags) {
+    if (tags.size() < static_cast<std::size_t>(m_n)) throw std::invalid_argument("tags too short");
+    m_tags = tags.data();
+  }
+  void set_tags(std::vector<std::uint32_t>& tags) { set_tags(std::span<std::uint32_t>(tags)); }
+
+  const float* coord_ptr(std::size_t d) const { return m_coords[d]; }
+  const float* weight_ptr() const { return m_weight; }
+  int* cluster_ptr() const { return m_cluster; }
+  const float* sigma_ptr(std::size_t d) const { return m_sigma[d]; }
+  const float* uncertainty_ptr() const { return m_u; }
+  const std::uint32_t* tags_ptr() const { return m_tags; }
+
+ private:
+  static int32_t check_n(int32_t n) {
+    if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
+    return n;
+  }
+  std::span<float> check_span(std::span<float> s) const {
+    if (s.size() < static_cast<std::size_t>(m_n)) throw std::invalid_argument("array too short");
+    return s;
+  }
+  int32_t m_n;
+  std::vector<float> m_own;
+  std::vector<int> m_own_idx;
+  std::array<float*, Ndim> m_coords{};
+  float* m_weight = nullptr;
+  int* m_cluster = nullptr;
+  std::array<float*, Ndim> m_sigma{};
+  float* m_u = nullptr;
+  std::uint32_t* m_tags = nullptr;
+  bool m_clustered = false;
+};
+
+// Device SoA (detail/DeviceViewPartition.hpp:14-20): coordinates, weights, and the
+// rho / nearest_higher / is_seed / cluster_index results; optional sigma, density
+// uncertainty and tags. All device memory, owned.
+template <std::size_t Ndim, class T = float>
+class PointsDevice {
+  static_assert(std::is_same_v<std::remove_cv_t<T>, float>,
+                "the MI355X device path is single precision");
+
+ public:
+  struct View {
+    std::array<float*, Ndim> m_coords;
+    float* m_weight;
+    int32_t* m_cluster_index;
+    int32_t* m_is_seed;
+    float* m_rho;
+    int32_t* m_nearest_higher;
+    float* m_delta;
+    int32_t m_n;
+    std::span<int32_t> cluster_index() const { return {m_cluster_index, std::size_t(m_n)}; }
+    std::span<int32_t> is_seed() const { return {m_is_seed, std::size_t(m_n)}; }
+    std::span<float> rho() const { return {m_rho, std::size_t(m_n)}; }
+    std::span<int32_t> nearest_higher() const { return {m_nearest_higher, std::size_t(m_n)}; }
+    std::span<float> delta() const { return {m_delta, std::size_t(m_n)}; }
+    int32_t size() const { return m_n; }
+  };
+
+  PointsDevice(Queue& q, int32_t n) : m_q{q}, m_n{n} {
+    if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
+    const std::size_t nn = static_cast<std::size_t>(n);
+    m_block = alloc<float>((Ndim + 3) * nn);
+    m_ints = alloc<int32_t>(3 * nn);
+    for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = m_block.get() + d * nn;
+    m_view.m_weight = m_block.get() + Ndim * nn;
+    m_view.m_rho = m_block.get() + (Ndim + 1) * nn;
+    m_view.m_delta = m_block.get() + (Ndim + 2) * nn;
+    m_view.m_cluster_index = m_ints.get();
+    m_view.m_is_seed = m_ints.get() + nn;
+    m_view.m_nearest_higher = m_ints.get() + 2 * nn;
+    m_view.m_n = n;
+  }
+  int32_t size() const { return m_n; }
+  const View& view() const { return m_view; }
+  std::span<float> coords(std::size_t d) const {
+    if (d >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
+    return {m_view.m_coords[d], std::size_t(m_n)};
+  }
+  std::span<float> weights() const { return {m_view.m_weight, std::size_t(m_n)}; }
+  std::span<int32_t> clusterIndexes() const { return m_view.cluster_index(); }
+
+  void set_sigma(std::size_t dim, std::span<float> device_sigma) {
+    if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to set_sigma.");
+    m_sigma[dim] = device_sigma.data();
+  }
+  void set_density_uncertainty(std::span<float> device_u) { m_u = device_u.data(); }
+  void set_tags(std::span<std::uint32_t> device_tags) { m_tags = device_tags.data(); }
+
+  // Copies host sigma / uncertainty / tags into device buffers owned by these points.
+  void upload_optionals(const PointsHost<Ndim, float>& h) {
+    const std::size_t nn = std::size_t(m_n);
+    if (h.sigma_ptr(0)) {
+      m_sig_own = alloc<float>(Ndim * nn);
+      for (std::size_t d = 0; d < Ndim; ++d) {
+        if (!h.sigma_ptr(d)) throw std::invalid_argument("sigma must be set in every dimension");
+        detail::check(clue_memcpy_h2d(m_q.ctx(), m_sig_own.get() + d * nn, h.sigma_ptr(d), 4 * nn));
+        m_sigma[d] = m_sig_own.get() + d * nn;
+      }
+    }
+    if (h.uncertainty_ptr()) {
+      m_u_own = alloc<float>(nn);
+      detail::check(clue_memcpy_h2d(m_q.ctx(), m_u_own.get(), h.uncertainty_ptr(), 4 * nn));
+      m_u = m_u_own.get();
+    }
+    if (h.tags_ptr()) {
+      m_tags_own = alloc<std::uint32_t>(nn);
+      detail::check(clue_memcpy_h2d(m_q.ctx(), m_tags_own.get(), h.tags_ptr(), 4 * nn));
+      m_tags = m_tags_own.get();
+    }
+  }
+
+  clue_points c_points(int32_t offset = 0, int32_t n = -1) const {
+    clue_points p{};
+    p.n = n < 0 ? m_n : n;
+    for (std::size_t d = 0; d < Ndim; ++d) {
+      p.coords[d] = m_view.m_coords[d] + offset;
+      p.sigma[d] = m_sigma[d] ? m_sigma[d] + offset : nullptr;
+    }
+    p.weight = m_view.m_weight + offset;
+    p.density_uncertainty = m_u ? m_u + offset : nullptr;
+    p.tags = m_tags ? m_tags + offset : nullptr;
+    return p;
+  }
+  clue_results c_results() const {
+    clue_results r{};
+    r.cluster_index = m_view.m_cluster_index;
+    r.is_seed = m_view.m_is_seed;
+    r.rho = m_view.m_rho;
+    r.nearest_higher = m_view.m_nearest_higher;
+    r.delta = m_view.m_delta;
+    return r;
+  }
+  void mark_clustered() { m_clustered = true; }
+  bool clustered() const { return m_clustered; }
+
+  std::size_t n_clusters() const {
+    std::vector<int32_t> h(static_cast<std::size_t>(m_n));
+    detail::check(clue_memcpy_d2h(m_q.ctx(), h.data(), m_view.m_cluster_index, 4 * h.size()));
+    int32_t m = -1;
+    for (int32_t v : h) m = std::max(m, v);
+    return static_cast<std::size_t>(m + 1);
+  }
+
+ private:
+  template <class U>
+  std::unique_ptr<U, detail::DeviceDeleter<U>> alloc(std::size_t count) {
+    void* p = nullptr;
+    detail::check(clue_malloc(m_q.ctx(), count * sizeof(U), &p));
+    return std::unique_ptr<U, detail::DeviceDeleter<U>>(static_cast<U*>(p),
+                                                        detail::DeviceDeleter<U>{m_q.ctx()});
+  }
+  Queue m_q;
+  int32_t m_n;
+  std::unique_ptr<float, detail::DeviceDeleter<float>> m_block;
+  std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_ints;
+  std::unique_ptr<float, detail::DeviceDeleter<float>> m_sig_own, m_u_own;
+  std::unique_ptr<std::uint32_t, detail::DeviceDeleter<std::uint32_t>> m_tags_own;
+  View m_view{};
+  std::array<float*, Ndim> m_sigma{};
+  float* m_u = nullptr;
+  std::uint32_t* m_tags = nullptr;
+  bool m_clustered = false;
+};
+
+// copyToDevice / copyToHost (data_structures/detail/PointsConversion.hpp:21-111).
+template <std::size_t Ndim>
+void copyToDevice(Queue& q, PointsDevice<Ndim, float>& d, const PointsHost<Ndim, float>& h) {
+  if (d.size() != h.size()) throw std::invalid_argument("host and device sizes differ");
+  const std::size_t nn = std::size_t(h.size());
+  for (std::size_t k = 0; k < Ndim; ++k)
+    detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_coords[k], h.coord_ptr(k), 4 * nn));
+  detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_weight, h.weight_ptr(), 4 * nn));
+  d.upload_optionals(h);
+}
+template <std::size_t Ndim>
+void copyToHost(Queue& q, PointsHost<Ndim, float>& h, const PointsDevice<Ndim, float>& d) {
+  if (d.size() != h.size()) throw std::invalid_argument("host and device sizes differ");
+  detail::check(clue_memcpy_d2h(q.ctx(), h.cluster_ptr(), d.view().m_cluster_index,
+                                4 * std::size_t(h.size())));
+  h.mark_clustered();
+}
+
+inline AssociationMapHost get_clusters(std::span<const int32_t> cluster_index) {
+  return AssociationMapHost::from_associations(cluster_index);
+}
+template <std::size_t Ndim>
+AssociationMapHost get_clusters(const PointsHost<Ndim, float>& h) {
+  return get_clusters(h.clusterIndexes());
+}
+
+// ---- Clusterer (core/Clusterer.hpp:37-312) -----------------------------------------------
+template <std::size_t Ndim, class DataType = float>
+class Clusterer {
+  static_assert(std::is_same_v<DataType, float>, "the MI355X device path is single precision");
+  static_assert(Ndim >= 1 && Ndim <= 10, "This library only works up to 10 dimensions");
+
+ public:
+  using value_type = float;
+
+  Clusterer(float dc, float rhoc, std::optional<float> dm = std::nullopt,
+            std::optional<float> seed_dc = std::nullopt)
+      : m_q{std::make_shared<Queue>(0)} {
+    setParameters(dc, rhoc, dm, seed_dc);
+  }
+  Clusterer(Queue& q, float dc, float rhoc, std::optional<float> dm = std::nullopt,
+            std::optional<float> seed_dc = std::nullopt)
+      : m_q{std::make_shared<Queue>(q)} {
+    setParameters(dc, rhoc, dm, seed_dc);
+  }
+
+  // core/detail/Clusterer.hpp:75-89
+  void setParameters(float dc, float rhoc, std::optional<float> dm = std::nullopt,
+                     std::optional<float> seed_dc = std::nullopt) {
+    m_dc = dc;
+    m_rhoc = rhoc;
+    m_dm = dm.value_or(dc);
+    m_seed_dc = seed_dc.value_or(dc);
+    if (!(m_dc > 0.f) || !(m_rhoc >= 0.f) || !(m_dm > 0.f) || !(m_seed_dc > 0.f))
+      throw std::invalid_argument("Invalid clustering parameters. The parameters must be positive.");
+  }
+
+  template <std::ranges::contiguous_range R>
+    requires std::integral<std::ranges::range_value_t<R>>
+  void setWrappedCoordinates(const R& wrapped) {
+    std::size_t d = 0;
+    for (auto w : wrapped) {
+      if (d == Ndim) break;
+      m_wrapped[d++] = static_cast<uint8_t>(w);
+    }
+  }
+  template <std::integral... A>
+  void setWrappedCoordinates(A... wrapped) {
+    m_wrapped = {static_cast<uint8_t>(wrapped)...};
+  }
+
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{.5f}) {
+    PointsDevice<Ndim, float> d(q, h.size());
+    make_clusters(q, h, d, metric, kernel);
+  }
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(PointsHost<Ndim, float>& h, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{.5f}) {
+    Queue q(0);
+    make_clusters(q, h, metric, kernel);
+  }
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, PointsDevice<Ndim, float>& d,
+                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{.5f}) {
+    copyToDevice(q, d, h);
+    make_clusters(q, d, metric, kernel);
+    copyToHost(q, h, d);
+  }
+  // Device-resident per event (core/detail/Clusterer.hpp:143-151).
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(Queue& q, PointsDevice<Ndim, float>& d, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{.5f}) {
+    clue_params p = params(metric, kernel);
+    clue_points pts = d.c_points();
+    clue_results res = d.c_results();
+    detail::check(clue_make_clusters(q.ctx(), &p, &pts, &res));
+    m_last = q;
+    m_event_of_cluster.clear();
+    d.mark_clustered();
+  }
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, PointsDevice<Ndim, float>& d,
+                     std::span<const uint32_t> sizes, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{.5f}) {
+    copyToDevice(q, d, h);
+    make_clusters(q, d, sizes, metric, kernel);
+    copyToHost(q, h, d);
+  }
+  // Batched (core/detail/Clusterer.hpp:171-183, 286-350); every event is tiled alone.
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
+            concepts::convolutional_kernel Kernel = FlatKernel<float>>
+  void make_clusters(Queue& q, PointsDevice<Ndim, float>& d, std::span<const uint32_t> sizes,
+                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{.5f}) {
+    clue_params p = params(metric, kernel);
+    clue_points pts = d.c_points();
+    clue_results res = d.c_results();
+    void* eoc = nullptr;
+    detail::check(clue_malloc(q.ctx(), 4 * std::size_t(d.size()), &eoc));
+    int64_t nclusters = 0;
+    clue_status st = clue_make_clusters_batched(q.ctx(), &p, &pts, sizes.data(),
+                                                static_cast<int32_t>(sizes.size()), &res,
+                                                static_cast<int32_t*>(eoc), &nclusters);
+    if (st == CLUE_OK) {
+      m_event_of_cluster.assign(std::size_t(nclusters), 0);
+      if (nclusters > 0)
+        st = clue_memcpy_d2h(q.ctx(), m_event_of_cluster.data(), eoc, 4 * std::size_t(nclusters));
+    }
+    clue_free(q.ctx(), eoc);
+    detail::check(st);
+    m_last = q;
+    d.mark_clustered();
+  }
+
+  // getSeeds (core/detail/Clusterer.hpp:198-203): host copy of the seed indices.
+  std::vector<int32_t> getSeeds() const {
+    if (!m_last) throw std::runtime_error("Seeds are not available. Please run make_clusters first.");
+    int64_t n = 0;
+    detail::check(clue_get_n_clusters(m_last->ctx(), &n));
+    std::vector<int32_t> s(std::size_t(std::max<int64_t>(n, 0)));
+    detail::check(clue_get_seeds(m_last->ctx(), s.data(), n, &n));
+    return s;
+  }
+  AssociationMapHost getClusters(const PointsHost<Ndim, float>& h) const { return get_clusters(h); }
+  // getSampleAssociations (core/detail/Clusterer.hpp:221-241): event -> clusters.
+  AssociationMapHost getSampleAssociations(Queue&, const PointsHost<Ndim, float>&) const {
+    return AssociationMapHost::from_associations(m_event_of_cluster);
+  }
+  AssociationMapHost getSampleAssociations(Queue&, const PointsDevice<Ndim, float>&) const {
+    return AssociationMapHost::from_associations(m_event_of_cluster);
+  }
+
+ private:
+  template <class Metric, class Kernel>
+  clue_params params(const Metric& metric, const Kernel& kernel) const {
+    clue_params p{};
+    p.ndim = static_cast<int32_t>(Ndim);
+    p.dc = m_dc;
+    p.rhoc = m_rhoc;
+    p.dm = m_dm;
+    p.seed_dc = m_seed_dc;
+    for (std::size_t d = 0; d < Ndim; ++d) p.wrapped[d] = m_wrapped[d];
+    metric.fill(p);
+    kernel.fill(p);
+    return p;
+  }
+
+  std::shared_ptr<Queue> m_q;
+  std::optional<Queue> m_last;
+  float m_dc{}, m_rhoc{}, m_dm{}, m_seed_dc{};
+  std::array<uint8_t, Ndim> m_wrapped{};
+  std::vector<int32_t> m_event_of_cluster;
+};
+
+// read_csv (utils/detail/read_csv.hpp:22-104): header line, then Ndim coordinates and
+// the weight per row.
+template <std::size_t Ndim, class T = float>
+PointsHost<Ndim, T> read_csv(Queue& q, const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("Could not open file " + path);
+  std::string line;
+  std::getline(f, line);
+  std::vector<std::array<float, Ndim + 1>> rows;
+  while (std::getline(f, line)) {
+    if (line.empty()) continue;
+    std::stringstream ss(line);
+    std::array<float, Ndim + 1> r{};
+    std::string cell;
+    for (std::size_t k = 0; k < Ndim + 1; ++k) {
+      if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
+      r[k] = std::stof(cell);
+    }
+    rows.push_back(r);
+  }
+  PointsHost<Ndim, T> h(q, static_cast<int32_t>(rows.size()));
+  for (std::size_t i = 0; i < rows.size(); ++i) {
+    for (std::size_t d = 0; d < Ndim; ++d) h.coords(d)[i] = rows[i][d];
+    h.weights()[i] = rows[i][Ndim];
+  }
+  return h;
+}
+
+}  // namespace clue

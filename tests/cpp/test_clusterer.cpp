@@ -1,0 +1,181 @@
+// C++ drop-in checks: the reference's doctest cases (tests/test_clustering.cpp,
+// tests/test_clusterer.cpp, tests/test_mahalanobis.cpp, tests/test_batched_clustering.cpp)
+// written against include/cluestering_amd/CLUEstering.hpp. Needs a GPU; the pytest
+// wrapper tests/test_cpp_api.py runs the built binary.
+#include <cmath>
+#include <cstdio>
+#include <numbers>
+#include <string>
+#include <vector>
+
+#include "cluestering_amd/CLUEstering.hpp"
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond)                                                        \
+  do {                                                                     \
+    if (cond) {                                                            \
+      ++g_pass;                                                            \
+    } else {                                                               \
+      ++g_fail;                                                            \
+      std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);           \
+    }                                                                      \
+  } while (0)
+#define CHECK_THROWS(expr)                                                 \
+  do {                                                                     \
+    bool thrown_ = false;                                                  \
+    try {                                                                  \
+      (void)(expr);                                                        \
+    } catch (const std::exception&) {                                      \
+      thrown_ = true;                                                      \
+    }                                                                      \
+    CHECK(thrown_);                                                        \
+  } while (0)
+
+static std::string data_dir;
+
+static void test_constructor_throws() {
+  // tests/test_clusterer.cpp:46-55
+  auto q = clue::get_queue(0);
+  CHECK_THROWS(clue::Clusterer<2>(-1.f, 10.f));
+  CHECK_THROWS(clue::Clusterer<2>(1.f, -10.f));
+  CHECK_THROWS(clue::Clusterer<2>(q, -1.f, 10.f));
+  CHECK_THROWS(clue::Clusterer<2>(q, 1.f, -10.f));
+  CHECK_THROWS(clue::FlatKernel<float>(-1.f));
+  CHECK_THROWS(clue::ExponentialKernel<float>(-1.f, 1.f));
+  CHECK_THROWS(clue::GaussianKernel<float>(1.f, 0.f));
+}
+
+static void test_tag_tie_break() {
+  // tests/test_clustering.cpp:517-567
+  auto q = clue::get_queue(0u);
+  clue::Clusterer<2> algo(q, 1.f, 0.f, 1.f);
+  std::vector<float> input{0.f, 0.f, 0.f, 0.f, 1.f, 1.f};
+  std::vector<int> output(2);
+  auto read_back = [&](clue::PointsDevice<2>& d) {
+    std::vector<int32_t> nh(2), seed(2);
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), nh.data(), d.view().nearest_higher().data(), 8));
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), seed.data(), d.view().is_seed().data(), 8));
+    return std::make_pair(nh, seed);
+  };
+  {
+    clue::PointsHost<2> h(q, 2, input.data(), output.data());
+    clue::PointsDevice<2> d(q, 2);
+    algo.make_clusters(q, h, d);
+    auto [nh, seed] = read_back(d);
+    CHECK(nh[0] == 1 && nh[1] == -1 && seed[0] == 0 && seed[1] == 1);
+  }
+  {
+    clue::PointsHost<2> h(q, 2, input.data(), output.data());
+    std::vector<std::uint32_t> tags{100, 1};
+    h.set_tags(tags);
+    clue::PointsDevice<2> d(q, 2);
+    algo.make_clusters(q, h, d);
+    auto [nh, seed] = read_back(d);
+    CHECK(nh[1] == 0 && nh[0] == -1 && seed[1] == 0 && seed[0] == 1);
+  }
+}
+
+static void test_mahalanobis() {
+  // tests/test_mahalanobis.cpp:10-43
+  auto q = clue::get_queue(0u);
+  clue::Clusterer<1> algo(q, 1.f, 0.3f);
+  {
+    clue::PointsHost<1> p(q, 2);
+    p.coords(0)[0] = 2.f;
+    p.coords(0)[1] = 4.f;
+    p.weights()[0] = p.weights()[1] = 1.f;
+    algo.make_clusters(q, p);
+    CHECK(p.n_clusters() == 2);
+  }
+  {
+    clue::PointsHost<1> p(q, 2);
+    p.coords(0)[0] = 2.f;
+    p.coords(0)[1] = 4.f;
+    p.weights()[0] = p.weights()[1] = 1.f;
+    std::vector<float> sigma = {2.f, 2.f};
+    p.set_sigma(0, std::span(sigma));
+    algo.make_clusters(q, p, clue::metrics::Mahalanobis<1>{});
+    CHECK(p.n_clusters() == 1);
+  }
+}
+
+static void test_periodic() {
+  // tests/test_clustering.cpp:237-270
+  auto q = clue::get_queue(0u);
+  auto points = clue::read_csv<2, float>(q, data_dir + "/opposite_angles.csv");
+  clue::Clusterer<2> algo(q, .2f, 5.f, .2f);
+  algo.setWrappedCoordinates(0, 1);
+  algo.make_clusters(q, points,
+                     clue::metrics::PeriodicEuclidean<2>(
+                         std::array<float, 2>{0.f, 2.f * std::numbers::pi_v<float>}));
+  CHECK(points.n_clusters() == 1);
+}
+
+static void test_batched() {
+  // tests/test_batched_clustering.cpp:8-65 on 10 copies of data_1024.
+  auto q = clue::get_queue(0u);
+  auto one = clue::read_csv<2, float>(q, data_dir + "/data_1024.csv");
+  clue::Clusterer<2> algo(q, 1.3f, 10.f, 1.3f);
+  algo.make_clusters(q, one);
+  const auto truth = one.n_clusters();
+  clue::PointsHost<2> h(q, 10240);
+  for (int k = 0; k < 10; ++k)
+    for (int i = 0; i < 1024; ++i) {
+      h.coords(0)[k * 1024 + i] = one.coords(0)[i];
+      h.coords(1)[k * 1024 + i] = one.coords(1)[i];
+      h.weights()[k * 1024 + i] = one.weights()[i];
+    }
+  clue::PointsDevice<2> d(q, 10240);
+  std::vector<uint32_t> sizes(10, 1024);
+  algo.make_clusters(q, h, d, sizes);
+  CHECK(h.n_clusters() == truth * 10);
+  auto assoc = algo.getSampleAssociations(q, h);
+  CHECK(assoc.size() == 10);
+  for (int e = 0; e < 10; ++e) CHECK(assoc.count(e) == truth);
+  // Every event's labels equal the single-event labels shifted by e * truth.
+  bool same = true;
+  for (int k = 0; k < 10; ++k)
+    for (int i = 0; i < 1024; ++i) {
+      const int a = one.clusterIndexes()[i];
+      const int b = h.clusterIndexes()[k * 1024 + i];
+      same = same && (a < 0 ? b == -1 : b == a + k * static_cast<int>(truth));
+    }
+  CHECK(same);
+  auto seeds = algo.getSeeds();
+  CHECK(seeds.size() == truth * 10);
+}
+
+static void test_device_points_and_clusters() {
+  auto q = clue::get_queue(0u);
+  auto h = clue::read_csv<2, float>(q, data_dir + "/data_4096.csv");
+  clue::PointsDevice<2> d(q, h.size());
+  clue::copyToDevice(q, d, h);
+  clue::Clusterer<2> algo(q, 1.5f, 10.f, 1.5f);
+  algo.make_clusters(q, d);
+  clue::copyToHost(q, h, d);
+  auto clusters = algo.getClusters(h);
+  CHECK(clusters.size() == h.n_clusters());
+  std::size_t total = 0;
+  for (std::size_t c = 0; c < clusters.size(); ++c) total += clusters.count(int32_t(c));
+  std::size_t clustered = 0;
+  for (int v : h.clusterIndexes()) clustered += v >= 0;
+  CHECK(total == clustered);
+  CHECK(h.n_clusters() == 20);
+}
+
+int main(int argc, char** argv) {
+  data_dir = argc > 1 ? argv[1] : "tests/golden/reference_data";
+  try {
+    test_constructor_throws();
+    test_tag_tie_break();
+    test_mahalanobis();
+    test_periodic();
+    test_batched();
+    test_device_points_and_clusters();
+  } catch (const std::exception& e) {
+    std::printf("FAIL exception: %s\n", e.what());
+    ++g_fail;
+  }
+  std::printf("%d passed, %d failed\n", g_pass, g_fail);
+  return g_fail == 0 ? 0 : 1;
+}

@@ -1,0 +1,145 @@
+"""The Python drop-in surface (clusterer, CLUE_GPU_HIP.mainRun): input handling
+on the CPU, and runs on the GPU checked against the oracle."""
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from cluestering_amd import CLUEstering as clue
+from cluestering_amd import CLUE_Convolutional_Kernels as K
+from cluestering_amd import CLUE_GPU_HIP
+from conftest import DATA, read_points
+
+
+# ---- CPU: parsing and validation (CLUEstering.py:291-496) -------------------------------
+
+def test_read_data_formats_agree():
+    path = os.path.join(DATA, "data_1024.csv")
+    df = pd.read_csv(path)
+    c = clue.clusterer(1.5, 10.0)
+    c.read_data(path)
+    from_csv = c.clust_data.coords.copy()
+    assert from_csv.dtype == np.float32 and from_csv.shape == (3, 1024)
+    c.read_data(df)
+    np.testing.assert_array_equal(c.clust_data.coords, from_csv)
+    c.read_data({k: df[k].to_numpy() for k in df.columns})
+    np.testing.assert_array_equal(c.clust_data.coords, from_csv)
+    c.read_data([df.x0.to_numpy(np.float32), df.x1.to_numpy(np.float32),
+                 df.weight.to_numpy(np.float32)])
+    np.testing.assert_array_equal(c.clust_data.coords, from_csv)
+    assert c.n_dim == 2 and c.n_points == 1024 and c.wrapped == [0, 0]
+    assert c.coords.shape == (2, 1024) and c.weight.shape == (1024,)
+
+
+def test_read_data_errors():
+    c = clue.clusterer(1.5, 10.0)
+    with pytest.raises(ValueError):
+        c.read_data([[1.0, 2.0]])
+    with pytest.raises(ValueError):
+        c.read_data([[1.0]] * 12)
+    with pytest.raises(ValueError):
+        c.read_data("points.txt")
+
+
+def test_choose_kernel_and_metric():
+    c = clue.clusterer(1.5, 10.0)
+    c.choose_kernel("exp", [0.5, 1.0])
+    assert isinstance(c._kernel, K.ExponentialKernel)
+    c.choose_kernel("gaus", [1.0, 2.0])
+    assert isinstance(c._kernel, K.GaussianKernel)
+    with pytest.raises(ValueError):
+        c.choose_kernel("flat", [1.0, 2.0])
+    with pytest.raises(ValueError):
+        c.choose_kernel("box", [1.0])
+    c.choose_metric("manhattan")
+    assert c._metric.tag == "manhattan"
+    with pytest.raises(ValueError):
+        c.choose_metric("weighted_euclidean")
+    with pytest.raises(ValueError):
+        c.choose_metric("cosine")
+
+
+def test_defaults_follow_reference():
+    c = clue.clusterer(1.3, 5.0)
+    assert c._outlier_distance == 1.3 and c._seeding_distance == 1.3
+    assert isinstance(c._kernel, K.FlatKernel) and c._kernel.flat == 0.5
+    c.set_params(2.0, 3.0, 4.0)
+    assert (c._density_radius, c._min_density, c._outlier_distance, c._seeding_distance) == \
+        (2.0, 3.0, 4.0, 2.0)
+    with pytest.raises(ValueError):
+        c.read_data(os.path.join(DATA, "data_1024.csv"))
+        c.run_clue(backend="cpu serial")
+
+
+def test_mainrun_rejects_more_than_ten_dimensions(capsys):
+    res = np.zeros(4, np.int32)
+    CLUE_GPU_HIP.mainRun(1.0, 1.0, 1.0, 1.0, [0] * 11, np.zeros((12, 4), np.float32), res,
+                         K.FlatKernel(0.5), 11, None, 4, 0, K.EuclideanMetric())
+    assert "10 dimensions" in capsys.readouterr().out
+
+
+# ---- GPU: results against the oracle -------------------------------------------------------
+
+def _oracle(oracle, c, w, **kw):
+    return oracle.run(oracle.Config(**kw), c, w)
+
+
+@pytest.mark.gpu
+def test_fit_matches_oracle(oracle_mod):
+    path = os.path.join(DATA, "data_4096.csv")
+    cl = clue.clusterer(1.5, 10.0, 1.5)
+    labels = cl.fit_predict(path)
+    c, w = read_points("data_4096.csv")
+    ref = _oracle(oracle_mod, c, w, ndim=2, dc=1.5, rhoc=10, dm=1.5)
+    np.testing.assert_array_equal(labels, ref.cluster)
+    assert cl.n_clusters == ref.cluster.max() + 1
+    for k in range(cl.n_clusters):
+        np.testing.assert_array_equal(cl.cluster_points[k], np.nonzero(ref.cluster == k)[0])
+    np.testing.assert_array_equal(cl.points_per_cluster,
+                                  np.bincount(ref.cluster[ref.cluster >= 0]))
+
+
+@pytest.mark.gpu
+def test_fit_kernels_metrics_and_partial_dimensions(oracle_mod):
+    c, w = read_points("blob.csv")
+    data = [c[0], c[1], c[2], w]
+    cl = clue.clusterer(1.0, 5.0, 2.0)
+    cl.choose_kernel("gaus", [1.0, 1.0])
+    cl.choose_metric("manhattan")
+    labels = cl.fit_predict(data)
+    ref = _oracle(oracle_mod, c, w, ndim=3, dc=1.0, rhoc=5.0, dm=2.0, kernel="gaus",
+                  kparams=(1.0, 1.0), metric="manhattan")
+    np.testing.assert_array_equal(labels, ref.cluster)
+    # tests/test_partial_dimensional_clustering.py: dimensions=[0, 1] clusters the
+    # projection, equal to a 2-D run on those coordinates.
+    cl2 = clue.clusterer(1.0, 5.0, 2.0)
+    cl2.read_data(data)
+    cl2.run_clue(dimensions=[0, 1])
+    ref2 = _oracle(oracle_mod, np.ascontiguousarray(c[:2]), w, ndim=2, dc=1.0, rhoc=5.0, dm=2.0)
+    np.testing.assert_array_equal(cl2.cluster_ids, ref2.cluster)
+
+
+@pytest.mark.gpu
+def test_mainrun_batched_and_periodic(oracle_mod):
+    c, w = read_points("data_1024.csv")
+    data = np.ascontiguousarray(np.vstack([np.tile(c, (1, 3)), np.tile(w, 3)]), np.float32)
+    res = np.zeros(3072, np.int32)
+    CLUE_GPU_HIP.mainRun(1.3, 10.0, 1.3, 1.3, [0, 0], data, res, K.FlatKernel(0.5), 2,
+                         np.array([1024, 1024, 1024], np.uint32), 3072, 0, K.EuclideanMetric())
+    ref = oracle_mod.run_batched(oracle_mod.Config(ndim=2, dc=1.3, rhoc=10, dm=1.3),
+                                 data[:2], data[2], [1024] * 3)
+    np.testing.assert_array_equal(res, ref.cluster)
+    c, w = read_points("opposite_angles.csv")
+    cl = clue.clusterer(0.2, 5.0, 0.2)
+    cl.read_data([c[0], c[1], w], wrapped_coords=[0, 1])
+    cl.choose_metric("periodic_euclidean", [0.0, 2 * np.pi])
+    cl.run_clue()
+    assert cl.n_clusters == 1
+
+
+@pytest.mark.gpu
+def test_list_devices(capsys):
+    CLUE_GPU_HIP.listDevices("gpu hip")
+    out = capsys.readouterr().out
+    assert "device 0" in out and "gfx950" in out
