@@ -64,8 +64,8 @@ struct DevBuf {
 // Workspace slots of one batch run.
 enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
-  S_NITEMS, S_ITEMS, S_SRHO, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX, S_SW,
-  S_SSIG, S_SU, S_STAG, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
+  S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX, S_SW,
+  S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
   S_NUM
 };
 
@@ -200,6 +200,13 @@ RunParams make_run_params(const clue_params* p, const clue_points* pts, int n_ev
   rp.kp1 = p->kparam[1];
   // 2 * sigma * sigma, evaluated left to right (ConvolutionalKernel.hpp:40).
   rp.gauss_den = (2.0f * p->kparam[0]) * p->kparam[0];
+  {
+    int ex = 0;
+    const float m = std::frexp(rp.gauss_den, &ex);
+    // Power of two with a normal, finite reciprocal.
+    rp.gauss_pow2 = (m == 0.5f && rp.gauss_den >= 0x1p-120f && rp.gauss_den <= 0x1p120f) ? 1 : 0;
+    rp.gauss_inv_den = rp.gauss_pow2 ? std::ldexp(1.0f, 1 - ex) : 0.0f;
+  }
   for (int d = 0; d < kMaxDim; ++d) {
     rp.wrapped[d] = d < p->ndim ? (p->wrapped[d] ? 1 : 0) : 0;
     rp.mparam[d] = d < p->ndim ? p->mparam[d] : 0.0f;
@@ -325,6 +332,8 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
       {S_NITEMS, sizeof(int32_t) * 4},
       {S_ITEMS, sizeof(Item) * bp.max_items},
       {S_SRHO, sizeof(float) * n},
+      {S_SKEY, sizeof(uint64_t) * n},
+      {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(float) * n : 0},
       {S_SNH, sizeof(int32_t) * n},
       {S_SDELTA, sizeof(float) * n},
       {S_FLAG, sizeof(int32_t) * n},
@@ -369,6 +378,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   b.n_items = ws.buf[S_NITEMS].as<int32_t>();
   b.items = ws.buf[S_ITEMS].as<Item>();
   b.srho = ws.buf[S_SRHO].as<float>();
+  b.skey = ws.buf[S_SKEY].as<uint64_t>();
   b.snh = ws.buf[S_SNH].as<int32_t>();
   b.sdelta = ws.buf[S_SDELTA].as<float>();
   b.flag = ws.buf[S_FLAG].as<int32_t>();
@@ -381,6 +391,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
     b.sorted.sigma[d] = (has_sigma && d < D) ? ws.buf[S_SSIG].as<float>() + stride * d : nullptr;
   }
   b.sorted.w = ws.buf[S_SW].as<float>();
+  b.sorted.fw = p->kernel == CLUE_KERNEL_FLAT ? ws.buf[S_SFW].as<float>() : nullptr;
   b.sorted.u = pts->density_uncertainty ? ws.buf[S_SU].as<float>() : nullptr;
   b.sorted.tag = ws.buf[S_STAG].as<uint32_t>();
   b.sorted.perm = ws.buf[S_PERM].as<int32_t>();

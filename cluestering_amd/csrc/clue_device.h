@@ -58,9 +58,11 @@ __device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool val
 // same bits.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float expf_cw(float x) {
-  if (x != x) return x;
-  if (x > 0x1.62e42ep+6f) return __builtin_inff();
-  if (x < -0x1.9fe368p+6f) return 0.0f;
+  // Branch-free: the polynomial runs on x clamped to [LO, HI] and the out-of-range
+  // results are selected afterwards, which gives the same value for every x.
+  const float lo = -0x1.9fe368p+6f, hi = 0x1.62e42ep+6f;
+  const float xin = x;
+  x = __builtin_fminf(__builtin_fmaxf(x, lo), hi);
   const float log2e = 0x1.715476p+0f;
   const float ln2_hi = 0x1.62e400p-1f;
   const float ln2_lo = 0x1.7f7d1cp-20f;
@@ -75,7 +77,25 @@ __device__ __forceinline__ float expf_cw(float x) {
   p = __builtin_fmaf(p, r, 0x1.000000p-1f);
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
-  return __builtin_ldexpf(p, static_cast<int>(k));
+  float e = __builtin_ldexpf(p, static_cast<int>(k));
+  e = (xin > hi) ? __builtin_inff() : e;
+  e = (xin < lo) ? 0.0f : e;
+  return (xin != xin) ? xin : e;
+}
+
+// Ordering key of (rho, tag) such that, for a query point i and a candidate j,
+// key(j, candidate) > key(i, query) exactly when j is "higher" than i in the
+// reference's sense (ClusteringKernels.hpp:180-182):
+//   rho_j > rho_i || (rho_j == rho_i && rho_j > 0 && tag_j > tag_i).
+// The high word orders rho (-0 folded onto +0, as the float comparison does);
+// the low word holds the tag only for rho > 0. NaN never compares higher: as a
+// candidate its key is the minimum, as a query the maximum.
+__device__ __forceinline__ uint64_t rho_tag_key(float r, uint32_t tag, bool query) {
+  if (r != r) return query ? ~0ull : 0ull;
+  if (r == 0.0f) r = 0.0f;
+  const uint32_t b = __float_as_uint(r);
+  const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return (static_cast<uint64_t>(o) << 32) | (r > 0.0f ? tag : 0u);
 }
 
 // static_cast<int>(float) with the x86 semantics the reference's serial backend

@@ -32,6 +32,7 @@ struct PointArrays {
 struct SortedArrays {
   float* x[kMaxDim];
   float* w;
+  float* fw;      // flat kernel only: flat value times weight
   float* sigma[kMaxDim];
   float* u;
   uint32_t* tag;  // user tags, or the original index when no tags are given
@@ -61,6 +62,7 @@ struct BatchBuffers {
   int32_t* n_items;    // [1]
   Item* items;         // [max_items]
   float* srho;
+  uint64_t* skey;      // (rho, tag) ordering key of every sorted point
   int32_t* snh;
   float* sdelta;
   int32_t* flag;       // [n_total]

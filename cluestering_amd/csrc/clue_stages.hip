@@ -162,7 +162,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs a, const Run
   if (s >= n) return;
   const int i = perm[s];
   for (int d = 0; d < rp.D; ++d) a.out.x[d][s] = a.in.x[d][i];
-  a.out.w[s] = a.in.w[i];
+  const float wi = a.in.w[i];
+  a.out.w[s] = wi;
+  if (a.out.fw) a.out.fw[s] = rp.kp0 * wi;  // flat kernel: K * w for j != i
   if (rp.has_sigma)
     for (int d = 0; d < rp.D; ++d) a.out.sigma[d][s] = a.in.sigma[d][i];
   if (rp.has_u) a.out.u[s] = a.in.u[i];
@@ -344,15 +346,48 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 }
 
 // ---------------------------------------------------------------------------
+// Kernel values (ConvolutionalKernel.hpp:21-57) for a candidate within the radius.
+// K_GAUSS2 is the gaussian whose 2*sigma*sigma is a power of two: there division
+// by it and multiplication by its reciprocal are the same correctly rounded
+// operation on the same exact value, so the cheaper one is used.
+// ---------------------------------------------------------------------------
+enum { K_GAUSS2 = 3 };
+
+template <int M, int K>
+__device__ __forceinline__ float kernel_value(float v, const RunParams& rp) {
+  if constexpr (K == K_FLAT) {
+    return rp.kp0;
+  } else if constexpr (K == K_EXP) {
+    return rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
+  } else if constexpr (K == K_GAUSS) {
+    const float dd = finish_distance<M>(v);
+    return rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
+  } else {
+    const float dd = finish_distance<M>(v);
+    return rp.kp1 * expf_cw((-dd * dd) * rp.gauss_inv_den);
+  }
+}
+
+// Candidates are walked in blocks of U so that their wave-uniform data arrives
+// in a few wide scalar loads.
+template <int D>
+struct Unroll {
+  static constexpr int U = D <= 3 ? 8 : 4;
+};
+
+// ---------------------------------------------------------------------------
 // (b) Local density: rho_i = sum over visited j of ((float)(d <= dc) * K) * w_j
-// (ClusteringKernels.hpp:53-69). Candidates beyond dc add +0 and are skipped.
+// (ClusteringKernels.hpp:53-69). Candidates beyond dc add +0 and are skipped;
+// the additions keep the candidate order.
 // ---------------------------------------------------------------------------
 template <int D, int M, int K>
 __global__ __launch_bounds__(kBlock) void k_density(
     const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
-    const float* __restrict__ sw, const float* __restrict__ ssig, float* __restrict__ srho) {
+    const float* __restrict__ sw, const float* __restrict__ sfw, const float* __restrict__ ssig,
+    const uint32_t* __restrict__ stag, float* __restrict__ srho, uint64_t* __restrict__ skey) {
+  constexpr int U = Unroll<D>::U;
   const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
@@ -377,57 +412,70 @@ __global__ __launch_bounds__(kBlock) void k_density(
     }
   }
   const float thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
-  const float f = rp.kp0;
   float rho = 0.0f;
+
+  // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
+  auto add = [&](float v, int j, float thr, bool own) {
+    if constexpr (K == K_FLAT) {
+      // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
+      // the reference's addition of a zero term.
+      const float next = rho + ((own && j == s) ? sw[j] : sfw[j]);
+      rho = (v <= thr) ? next : rho;
+    } else {
+      if (v <= thr) {
+        const float kv = (own && j == s) ? 1.0f : kernel_value<M, K>(v, rp);
+        rho += kv * sw[j];
+      }
+    }
+  };
+  auto walk = [&](int j0, int j1, float thr, bool own) {
+    int j = j0;
+    for (; j + U <= j1; j += U) {
+      float v[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
+#pragma unroll
+      for (int k = 0; k < U; ++k) add(v[k], j + k, thr, own);
+    }
+    for (; j < j1; ++j) add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, thr, own);
+  };
 
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
     const int j0 = toff[t];
     const int j1 = toff[t + 1];
     const float thr = in ? thr_in : -1.0f;
-    if (t == it.tile) {
-      for (int j = j0; j < j1; ++j) {
-        const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
-        if (v <= thr) {
-          float kv;
-          if (j == s) {
-            kv = 1.0f;
-          } else if constexpr (K == K_FLAT) {
-            kv = f;
-          } else if constexpr (K == K_EXP) {
-            kv = rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
-          } else {
-            const float dd = finish_distance<M>(v);
-            kv = rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
-          }
-          rho += kv * sw[j];
-        }
-      }
-    } else {
-      for (int j = j0; j < j1; ++j) {
-        const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
-        if constexpr (K == K_FLAT) {
-          const float term = f * sw[j];
-          rho += (v <= thr) ? term : 0.0f;
-        } else if (v <= thr) {
-          float kv;
-          if constexpr (K == K_EXP) {
-            kv = rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
-          } else {
-            const float dd = finish_distance<M>(v);
-            kv = rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
-          }
-          rho += kv * sw[j];
-        }
-      }
-    }
+    if (t == it.tile)
+      walk(j0, j1, thr, true);
+    else
+      walk(j0, j1, thr, false);
   });
-  if (active) srho[s] = rho;
+  if (active) {
+    srho[s] = rho;
+    skey[s] = rho_tag_key(rho, stag[s], false);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // (c) Nearest-higher (ClusteringKernels.hpp:164-206, 247-289). The search box uses
 // dm; the acceptance radius is seed_dc for points at or above rhoc*u, else dm.
+// A candidate is "higher" when its (rho, tag) key exceeds the point's. Only
+// candidates at or below the current best distance can change the result; the
+// running threshold below over-approximates that bound, and every candidate that
+// passes it goes through the reference's exact comparison.
 // ---------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ float prune_threshold(float eff_thr, float delta) {
+  if constexpr (MetricTraits<M>::kSqrt) {
+    // sqrt_rn(t) <= delta implies t <= delta^2 (1 + 2^-22); delta*delta*(1 + 2^-20)
+    // rounded bounds that from above for delta >= 2^-60 (no underflow).
+    if (!(delta >= 0x1p-60f)) return eff_thr;
+    const float b = (delta * delta) * 0x1.00001p+0f;
+    return b < eff_thr ? b : eff_thr;
+  } else {
+    return delta < eff_thr ? delta : eff_thr;
+  }
+}
+
 template <int D, int M>
 __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
@@ -435,7 +483,8 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
     const float* __restrict__ ssig, const float* __restrict__ su,
     const uint32_t* __restrict__ stag, const float* __restrict__ srho,
-    int32_t* __restrict__ snh, float* __restrict__ sdelta) {
+    const uint64_t* __restrict__ skey, int32_t* __restrict__ snh, float* __restrict__ sdelta) {
+  constexpr int U = Unroll<D>::U;
   const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
@@ -451,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
   }
   const float rho_i = active ? srho[s] : 0.0f;
-  const uint32_t tag_i = active ? stag[s] : 0u;
+  const uint64_t key_i = active ? rho_tag_key(rho_i, stag[s], true) : ~0ull;
   const float ui = (active && rp.has_u) ? su[s] : 1.0f;
   const float eff_min = rp.rhoc * ui;
   const bool use_seed = rho_i >= eff_min;
@@ -472,26 +521,44 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
   int nh = -1;
   float rho_nh = 0.0f;
   uint32_t tag_nh = 0u;
+  float bound = eff_thr;  // prune_threshold(eff_thr, delta)
+
+  // The reference's acceptance test, in candidate order.
+  auto consider = [&](float v, int j) {
+    const float dist = finish_distance<M>(v);
+    const float rj = srho[j];
+    const uint32_t tj = stag[j];
+    if (dist <= eff &&
+        ((dist < delta) ||
+         ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+      delta = dist;
+      nh = j;
+      rho_nh = rj;
+      tag_nh = tj;
+      bound = prune_threshold<M>(eff_thr, delta);
+    }
+  };
 
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
     const int j0 = toff[t];
     const int j1 = toff[t + 1];
-    const float thr = in ? eff_thr : -1.0f;
-    for (int j = j0; j < j1; ++j) {
-      const float rj = srho[j];
-      const uint32_t tj = stag[j];
-      const bool higher = (rj > rho_i) || ((rj == rho_i) && (rj > 0.0f) && (tj > tag_i));
-      const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
-      if (higher && v <= thr) {
-        const float dist = finish_distance<M>(v);
-        if ((dist < delta) ||
-            ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh))))) {
-          delta = dist;
-          nh = j;
-          rho_nh = rj;
-          tag_nh = tj;
-        }
+    if (!in) return;  // lanes whose box does not hold this tile idle through it
+    int j = j0;
+    for (; j + U <= j1; j += U) {
+      float v[U];
+      bool pass[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
+        pass[k] = (skey[j + k] > key_i) & (v[k] <= bound);
       }
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (pass[k] && v[k] <= bound) consider(v[k], j + k);
+    }
+    for (; j < j1; ++j) {
+      const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
+      if ((skey[j] > key_i) & (v <= bound)) consider(v, j);
     }
   });
   if (active) {
@@ -568,7 +635,7 @@ static void launch_density_dmk(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
                      b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
-                     b.sorted.sigma[0], b.srho);
+                     b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.srho, b.skey);
 }
 
 template <int D, int M>
@@ -576,7 +643,12 @@ static void launch_density_dm(const NeighbourLaunch& L) {
   switch (L.rp->kernel) {
     case K_FLAT: launch_density_dmk<D, M, K_FLAT>(L); break;
     case K_EXP: launch_density_dmk<D, M, K_EXP>(L); break;
-    default: launch_density_dmk<D, M, K_GAUSS>(L); break;
+    default:
+      if (L.rp->gauss_pow2)
+        launch_density_dmk<D, M, K_GAUSS2>(L);
+      else
+        launch_density_dmk<D, M, K_GAUSS>(L);
+      break;
   }
 }
 
@@ -585,7 +657,8 @@ static void launch_nh_dm(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp,
                      b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
-                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.snh, b.sdelta);
+                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.snh,
+                     b.sdelta);
 }
 
 template <int D>

@@ -53,6 +53,8 @@ struct RunParams {
   float seed_sq_max;    // same for seed_dc
   float kp0, kp1;       // kernel parameters (flat f | exp lambda, A | gauss sigma, A)
   float gauss_den;      // (2 * sigma) * sigma, as the reference evaluates it
+  float gauss_inv_den;  // 1 / gauss_den when gauss_den is a power of two
+  int32_t gauss_pow2;   // gauss_den is a power of two
   int32_t wrapped[kMaxDim];
   float mparam[kMaxDim];
 };

@@ -64,7 +64,9 @@ def test_batched_gaussian(gpu_ctx, oracle_mod):
 
 
 @pytest.mark.parametrize("kernel", [K.FlatKernel(0.5), K.ExponentialKernel(0.5, 1.0),
-                                    K.GaussianKernel(1.0, 1.0), K.FlatKernel(2.0)])
+                                    K.GaussianKernel(1.0, 1.0), K.FlatKernel(2.0),
+                                    K.GaussianKernel(0.7, 1.3), K.ExponentialKernel(1.7, 0.8),
+                                    K.GaussianKernel(0.5, 2.0)])
 @pytest.mark.parametrize("metric", [None, K.EuclideanMetric(), K.EuclideanMetric([1.0, 2.0]),
                                     K.ManhattanMetric(), K.ChebyshevMetric(),
                                     K.ChebyshevMetric([1.0, 2.0])])
