@@ -140,6 +140,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-event", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus > 1) are not "
@@ -147,6 +149,16 @@ def main():
         sys.exit(2)
 
     from cluestering_amd.device import Clusterer, Context, Plan, PointsDevice
+
+    if args.calibrate_copy:
+        from cluestering_amd import _capi
+        ctx = Context(0)
+        nbytes = args.calibrate_copy << 20
+        src, dst = ctx.empty(nbytes, np.uint8), ctx.empty(nbytes, np.uint8)
+        _capi.check(_capi.load().clue_debug_copy(ctx.handle, C.c_void_p(dst.ptr),
+                                                 C.c_void_p(src.ptr), nbytes))
+        print(json.dumps({"calibration_copy_bytes": nbytes}))
+        return
 
     w = _workload(args.workload)
     sizes = w.sizes if w.sizes is not None else np.array([w.n], np.uint32)

@@ -112,6 +112,7 @@ _SIGNATURES = {
     "clue_memset_d8": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int, C.c_size_t]),
     "clue_debug_math": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
+    "clue_debug_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
 }
 
 # Every symbol include/clue_amd.h declares.

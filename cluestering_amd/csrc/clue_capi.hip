@@ -207,6 +207,15 @@ RunParams make_run_params(const clue_params* p, const clue_points* pts, int n_ev
     rp.gauss_pow2 = (m == 0.5f && rp.gauss_den >= 0x1p-120f && rp.gauss_den <= 0x1p120f) ? 1 : 0;
     rp.gauss_inv_den = rp.gauss_pow2 ? std::ldexp(1.0f, 1 - ex) : 0.0f;
   }
+  // Within the radius the kernel's exp argument is -lambda*d (exp) or -d*d/(2s^2)
+  // (gauss), with 0 <= d <= dc; bound its magnitude with a 1% margin for rounding.
+  {
+    const double dc = p->dc;
+    double mag = 0.0;
+    if (p->kernel == CLUE_KERNEL_EXP) mag = static_cast<double>(p->kparam[0]) * dc;
+    if (p->kernel == CLUE_KERNEL_GAUSS) mag = dc * dc / static_cast<double>(rp.gauss_den);
+    rp.exp_in_range = (std::isfinite(mag) && mag * 1.01 < 100.0) ? 1 : 0;
+  }
   for (int d = 0; d < kMaxDim; ++d) {
     rp.wrapped[d] = d < p->ndim ? (p->wrapped[d] ? 1 : 0) : 0;
     rp.mparam[d] = d < p->ndim ? p->mparam[d] : 0.0f;
@@ -767,5 +776,17 @@ extern "C" clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a,
   ScopedDevice dev(ctx->device);
   clue_status st = clue_debug_math_impl(ctx->stream, n, a, b, out_sqrt, out_exp, out_div);
   if (st) return fail(st, "debug math kernel failed");
+  return CLUE_OK;
+}
+
+extern "C" clue_status clue_debug_copy_impl(void* stream, void* dst, const void* src,
+                                            size_t bytes);
+
+extern "C" clue_status clue_debug_copy(clue_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || !dst || !src) return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
+  if (bytes % 16) return fail(CLUE_ERR_INVALID_ARGUMENT, "bytes must be a multiple of 16");
+  ScopedDevice dev(ctx->device);
+  clue_status st = clue_debug_copy_impl(ctx->stream, dst, src, bytes);
+  if (st) return fail(st, "debug copy kernel failed");
   return CLUE_OK;
 }

@@ -20,7 +20,25 @@ __global__ void k_debug_math(int n, const float* __restrict__ a, const float* __
   if (out_div) out_div[i] = x / b[i];
 }
 
+// A plain streaming copy, the reference point for the HBM traffic counters.
+__global__ void k_debug_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+extern "C" clue_status clue_debug_copy_impl(void* stream, void* dst, const void* src,
+                                            size_t bytes) {
+  const int64_t n = static_cast<int64_t>(bytes / sizeof(uint4));
+  if (n <= 0) return CLUE_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_debug_copy, dim3(2048), dim3(256), 0, s, static_cast<const uint4*>(src),
+                     static_cast<uint4*>(dst), n);
+  if (hipGetLastError() != hipSuccess) return CLUE_ERR_HIP;
+  return hipStreamSynchronize(s) == hipSuccess ? CLUE_OK : CLUE_ERR_HIP;
+}
 
 extern "C" clue_status clue_debug_math_impl(void* stream, int32_t n, const float* a,
                                             const float* b, float* out_sqrt, float* out_exp,

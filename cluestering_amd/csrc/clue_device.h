@@ -57,12 +57,8 @@ __device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool val
 // and operation order as the CPU statement of the algorithm, so both give the
 // same bits.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float expf_cw(float x) {
-  // Branch-free: the polynomial runs on x clamped to [LO, HI] and the out-of-range
-  // results are selected afterwards, which gives the same value for every x.
-  const float lo = -0x1.9fe368p+6f, hi = 0x1.62e42ep+6f;
-  const float xin = x;
-  x = __builtin_fminf(__builtin_fmaxf(x, lo), hi);
+// The core for x in [LO, HI] = [-103.97, 88.72]: no range handling.
+__device__ __forceinline__ float expf_cw_core(float x) {
   const float log2e = 0x1.715476p+0f;
   const float ln2_hi = 0x1.62e400p-1f;
   const float ln2_lo = 0x1.7f7d1cp-20f;
@@ -77,7 +73,15 @@ __device__ __forceinline__ float expf_cw(float x) {
   p = __builtin_fmaf(p, r, 0x1.000000p-1f);
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
-  float e = __builtin_ldexpf(p, static_cast<int>(k));
+  return __builtin_ldexpf(p, static_cast<int>(k));
+}
+
+// Branch-free for every x: the core runs on x clamped to [LO, HI] and the
+// out-of-range results are selected afterwards.
+__device__ __forceinline__ float expf_cw(float x) {
+  const float lo = -0x1.9fe368p+6f, hi = 0x1.62e42ep+6f;
+  const float xin = x;
+  float e = expf_cw_core(__builtin_fminf(__builtin_fmaxf(x, lo), hi));
   e = (xin > hi) ? __builtin_inff() : e;
   e = (xin < lo) ? 0.0f : e;
   return (xin != xin) ? xin : e;

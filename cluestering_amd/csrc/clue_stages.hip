@@ -353,18 +353,25 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 // ---------------------------------------------------------------------------
 enum { K_GAUSS2 = 3 };
 
+// exp of an argument the host has shown to lie in [LO, HI] for every candidate
+// within the radius (rp.exp_in_range) skips the range handling; both forms give
+// the same value there.
+__device__ __forceinline__ float kernel_exp(float x, const RunParams& rp) {
+  return rp.exp_in_range ? expf_cw_core(x) : expf_cw(x);
+}
+
 template <int M, int K>
 __device__ __forceinline__ float kernel_value(float v, const RunParams& rp) {
   if constexpr (K == K_FLAT) {
     return rp.kp0;
   } else if constexpr (K == K_EXP) {
-    return rp.kp1 * expf_cw(-rp.kp0 * finish_distance<M>(v));
+    return rp.kp1 * kernel_exp(-rp.kp0 * finish_distance<M>(v), rp);
   } else if constexpr (K == K_GAUSS) {
     const float dd = finish_distance<M>(v);
-    return rp.kp1 * expf_cw(-dd * dd / rp.gauss_den);
+    return rp.kp1 * kernel_exp(-dd * dd / rp.gauss_den, rp);
   } else {
     const float dd = finish_distance<M>(v);
-    return rp.kp1 * expf_cw((-dd * dd) * rp.gauss_inv_den);
+    return rp.kp1 * kernel_exp((-dd * dd) * rp.gauss_inv_den, rp);
   }
 }
 
@@ -568,13 +575,16 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
 }
 
 // ---------------------------------------------------------------------------
-// (d) Back to input order; seed flags (ClusteringKernels.hpp:300-316).
+// (d) Seed flags (ClusteringKernels.hpp:300-316) and the results in input order.
+// snh is rewritten in place as the follower link of the label stage:
+// -2 for a seed, -1 for a point without nearest-higher, else the sorted
+// position of its nearest-higher.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_unpermute(
     const RunParams rp, int n, const int32_t* __restrict__ perm, const float* __restrict__ srho,
-    const int32_t* __restrict__ snh, const float* __restrict__ sdelta,
-    const float* __restrict__ su, float* __restrict__ rho, int32_t* __restrict__ nh,
-    float* __restrict__ delta, int32_t* __restrict__ is_seed, int32_t* __restrict__ flag) {
+    int32_t* __restrict__ snh, const float* __restrict__ sdelta, const float* __restrict__ su,
+    float* __restrict__ rho, int32_t* __restrict__ nh, float* __restrict__ delta,
+    int32_t* __restrict__ is_seed, int32_t* __restrict__ flag) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
@@ -587,35 +597,38 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
   if (delta) delta[i] = sdelta[s];
   is_seed[i] = seed;
   flag[i] = seed;
+  snh[s] = seed ? -2 : h;
 }
 
 // ---------------------------------------------------------------------------
 // (e) Labels: seeds take their rank in input order (the serial push_back order,
 // ClusteringKernels.hpp:310, 318-327); followers chase nearest_higher to a seed
-// or to a point without one (:329-343).
+// or to a point without one (:329-343). The chase runs in tile-sorted space,
+// where a chain stays among neighbouring tiles of one event. An event occupies
+// the same range of positions in input and sorted order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
-                                                   const int32_t* __restrict__ nh,
-                                                   const int32_t* __restrict__ is_seed,
+                                                   const int32_t* __restrict__ perm,
+                                                   const int32_t* __restrict__ link,
                                                    const int32_t* __restrict__ rank,
                                                    int32_t* __restrict__ cluster,
                                                    int32_t* __restrict__ seeds,
                                                    int32_t* __restrict__ event_of_cluster) {
   const Chunk ch = chunks[blockIdx.x];
-  for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
-    int c;
-    if (is_seed[i]) {
-      c = rank[i];
+  for (int s = ch.begin + threadIdx.x; s < ch.end; s += kBlock) {
+    int cur = s;
+    int nx = link[cur];
+    while (nx >= 0) {
+      cur = nx;
+      nx = link[cur];
+    }
+    const int i = perm[s];
+    const int c = (nx == -2) ? rank[perm[cur]] : -1;
+    cluster[i] = c;
+    if (cur == s && nx == -2) {
       seeds[c] = i;
       if (event_of_cluster) event_of_cluster[c] = ch.event;
-    } else if (nh[i] == -1) {
-      c = -1;
-    } else {
-      int cur = nh[i];
-      while (!is_seed[cur] && nh[cur] != -1) cur = nh[cur];
-      c = is_seed[cur] ? rank[cur] : -1;
     }
-    cluster[i] = c;
   }
 }
 
@@ -764,8 +777,9 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
                      out.delta, out.is_seed, bb.flag);
   scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
   mark(kStageLabels);
-  hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, out.nh,
-                     out.is_seed, bb.rank, out.cluster, bb.seeds, out.event_of_cluster);
+  hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks,
+                     bb.sorted.perm, bb.snh, bb.rank, out.cluster, bb.seeds,
+                     out.event_of_cluster);
   mark(kNumStages);
 }
 

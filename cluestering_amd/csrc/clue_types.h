@@ -55,6 +55,7 @@ struct RunParams {
   float gauss_den;      // (2 * sigma) * sigma, as the reference evaluates it
   float gauss_inv_den;  // 1 / gauss_den when gauss_den is a power of two
   int32_t gauss_pow2;   // gauss_den is a power of two
+  int32_t exp_in_range; // every kernel exp argument within the radius lies in [-103, 0]
   int32_t wrapped[kMaxDim];
   float mparam[kMaxDim];
 };

@@ -180,6 +180,10 @@ clue_status clue_memset_d8(clue_ctx* ctx, void* dst, int value, size_t bytes);
 clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a, const float* b,
                             float* out_sqrt, float* out_exp, float* out_div);
 
+/* A plain device copy of `bytes` (multiple of 16), the calibration point of the
+ * HBM traffic counters (FETCH_SIZE / WRITE_SIZE). No reference counterpart. */
+clue_status clue_debug_copy(clue_ctx* ctx, void* dst, const void* src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif
