@@ -140,6 +140,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-event", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-sweep", action="store_true",
+                    help="also sweep the served stream over 1k..50k events/s")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -268,6 +271,30 @@ def main():
 
     plan.close()
     ctx.close()
+
+    if not args.no_serve:
+        # Served stream: Poisson arrivals, ndim 2 or 3, sizes log-uniform [256, 262144],
+        # continuous batching under a 64 GB HBM budget; latency from arrival to labels.
+        from cluestering_amd.serving import Server, make_pool, run_stream
+        pool = make_pool(256)
+        rates = [1000.0, 2000.0, 5000.0, 10000.0, 20000.0, 50000.0] if args.serve_sweep \
+            else [2000.0]
+        served = []
+        with Server(n_lanes=2, hbm_budget_bytes=64 << 30) as srv:
+            run_stream(srv, pool, rate=2000.0, n_events=200, seed=99)  # warm the lane pools
+            for rate in rates:
+                n_ev = int(min(max(rate * 1.5, 400), 6000))
+                r = run_stream(srv, pool, rate=rate, n_events=n_ev)
+                row = {k: (round(v, 3) if isinstance(v, float) else v)
+                       for k, v in r.items() if k != "latency_ms"}
+                row["latency_ms"] = {k: round(v, 3) for k, v in r["latency_ms"].items()}
+                served.append(row)
+                print(f"[bench] served {rate:.0f}/s: {r['events_per_s']:.1f} events/s, "
+                      f"p50 {r['latency_ms']['p50']:.2f} ms, p99 {r['latency_ms']['p99']:.2f} ms",
+                      file=sys.stderr, flush=True)
+        out["served"] = {"workload": "Poisson arrivals, ndim 2|3, sizes log-uniform [256,262144], "
+                                     "2 lanes, 64 GB budget, latency arrival->labels on host",
+                         "rates": served}
     print(json.dumps(out))
 
 

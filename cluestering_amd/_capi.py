@@ -74,6 +74,38 @@ class Results(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("n_lanes", C.c_int32),
+        ("hbm_budget_bytes", C.c_int64),
+        ("max_batch_points", C.c_int64),
+        ("max_batch_events", C.c_int64),
+        ("staging_bytes", C.c_int64),
+    ]
+
+
+class ServerTiming(C.Structure):
+    _fields_ = [
+        ("submit_ns", C.c_int64),
+        ("launch_ns", C.c_int64),
+        ("done_ns", C.c_int64),
+        ("batch_events", C.c_int32),
+        ("reserved", C.c_int32),
+        ("batch_points", C.c_int64),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("events_submitted", C.c_int64),
+        ("events_done", C.c_int64),
+        ("points_done", C.c_int64),
+        ("batches", C.c_int64),
+        ("lane_points", C.c_int64),
+    ]
+
+
 _SIGNATURES = {
     "clue_last_error": (C.c_char_p, []),
     "clue_version": (C.c_char_p, []),
@@ -113,6 +145,14 @@ _SIGNATURES = {
     "clue_debug_math": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "clue_debug_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "clue_server_create": (C.c_int32, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "clue_server_destroy": (C.c_int32, [C.c_void_p]),
+    "clue_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p,
+                                       C.POINTER(C.c_uint64)]),
+    "clue_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_int64,
+                                     C.POINTER(ServerTiming)]),
+    "clue_server_get_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    "clue_now_ns": (C.c_int64, []),
 }
 
 # Every symbol include/clue_amd.h declares.

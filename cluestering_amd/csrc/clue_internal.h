@@ -80,6 +80,7 @@ struct ResultArrays {
   int32_t* nh;
   float* delta;
   int32_t* event_of_cluster;  // may be NULL
+  int32_t local_ids;          // number clusters per event from 0 (served events)
 };
 
 // Stage boundaries reported by the profiler.

@@ -31,6 +31,15 @@ enum { M_EUCLID = 0, M_WEUCLID = 1, M_PERIODIC = 2, M_MANHATTAN = 3, M_CHEBYSHEV
        M_WCHEBYSHEV = 5, M_MAHALANOBIS = 6 };
 enum { K_FLAT = 0, K_EXP = 1, K_GAUSS = 2 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Remapping block b to (b mod 8) * (nb / 8) + b / 8 gives every XCD a contiguous
+// range of work items (neighbouring tiles of the same events), so tile data is
+// shared in one L2 instead of being fetched by all eight. nb is a multiple of 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (b % kXcds) * (nb / kXcds) + b / kXcds;
+}
+
 template <int M>
 struct MetricTraits {
   // Metrics ending in a square root are compared through the squared sum against
@@ -395,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_density(
     const float* __restrict__ sw, const float* __restrict__ sfw, const float* __restrict__ ssig,
     const uint32_t* __restrict__ stag, float* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
-  const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -492,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const uint32_t* __restrict__ stag, const float* __restrict__ srho,
     const uint64_t* __restrict__ skey, int32_t* __restrict__ snh, float* __restrict__ sdelta) {
   constexpr int U = Unroll<D>::U;
-  const int item = blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -554,14 +563,19 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     for (; j + U <= j1; j += U) {
       float v[U];
       bool pass[U];
+      bool any = false;
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
         pass[k] = (skey[j + k] > key_i) & (v[k] <= bound);
+        any = any | pass[k];
       }
+      // Most blocks hold no candidate for any lane: one wave-wide test skips them.
+      if (ballot(any) != 0ull) {
 #pragma unroll
-      for (int k = 0; k < U; ++k)
-        if (pass[k] && v[k] <= bound) consider(v[k], j + k);
+        for (int k = 0; k < U; ++k)
+          if (pass[k] && v[k] <= bound) consider(v[k], j + k);
+      }
     }
     for (; j < j1; ++j) {
       const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
@@ -608,13 +622,17 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // the same range of positions in input and sorted order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
+                                                   const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ link,
                                                    const int32_t* __restrict__ rank,
                                                    int32_t* __restrict__ cluster,
                                                    int32_t* __restrict__ seeds,
-                                                   int32_t* __restrict__ event_of_cluster) {
+                                                   int32_t* __restrict__ event_of_cluster,
+                                                   int local_ids) {
   const Chunk ch = chunks[blockIdx.x];
+  // With per-event numbering, subtract the seeds of the earlier events.
+  const int base = local_ids ? rank[events[ch.event].p_off] : 0;
   for (int s = ch.begin + threadIdx.x; s < ch.end; s += kBlock) {
     int cur = s;
     int nx = link[cur];
@@ -624,10 +642,10 @@ __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chu
     }
     const int i = perm[s];
     const int c = (nx == -2) ? rank[perm[cur]] : -1;
-    cluster[i] = c;
+    cluster[i] = c >= 0 ? c - base : -1;
     if (cur == s && nx == -2) {
       seeds[c] = i;
-      if (event_of_cluster) event_of_cluster[c] = ch.event;
+      if (event_of_cluster) event_of_cluster[c] = events[ch.event].index;
     }
   }
 }
@@ -763,7 +781,8 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   L.rp = &rp;
   L.bb = &bb;
   L.cs = bb.sorted_stride;
-  L.grid = static_cast<unsigned>((bb.max_items + kWaves - 1) / kWaves);
+  const int64_t blocks = (bb.max_items + kWaves - 1) / kWaves;
+  L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
   mark(kStageDensity);
   launch_neighbour(L, true);
@@ -777,9 +796,9 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
                      out.delta, out.is_seed, bb.flag);
   scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
   mark(kStageLabels);
-  hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks,
+  hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, bb.events,
                      bb.sorted.perm, bb.snh, bb.rank, out.cluster, bb.seeds,
-                     out.event_of_cluster);
+                     out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
 

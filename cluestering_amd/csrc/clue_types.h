@@ -14,7 +14,8 @@ struct EventDesc {
   int32_t nper;    // tiles per dimension (SetupTiles.hpp:30-34)
   int32_t ntiles;  // nper^D
   int32_t t_base;  // first global tile key of the event
-  int32_t pad[3];
+  int32_t index;   // the caller's event index (empty events are not described)
+  int32_t pad[2];
 };
 
 // A contiguous range of points of one event, the unit of the per-point grids.

@@ -167,6 +167,50 @@ clue_status clue_plan_create(clue_ctx* ctx, const clue_params* params, const clu
 clue_status clue_plan_launch(clue_plan* plan); /* async on the context stream */
 clue_status clue_plan_destroy(clue_plan* plan);
 
+/* Continuous-batching server (no reference counterpart: the serving form of the
+ * batched make_clusters). Clients submit events from any thread; one worker thread
+ * folds every waiting event of the same parameters into the next launch on a free
+ * lane (stream with pooled device buffers), so uploads, kernels and downloads of
+ * consecutive batches overlap. Labels are numbered per event from 0, exactly as a
+ * single-event make_clusters numbers them. */
+typedef struct clue_server clue_server;
+typedef struct {
+  int32_t device;
+  int32_t n_lanes;          /* concurrent launches (streams); 0 selects 2 */
+  int64_t hbm_budget_bytes; /* device memory the lanes may use; 0 selects 64 GiB */
+  int64_t max_batch_points; /* points per launch; 0 derives it from the budget (max 16 Mi) */
+  int64_t max_batch_events; /* events per launch; 0 = no limit */
+  int64_t staging_bytes;    /* pinned host staging ring; 0 selects 1 GiB */
+} clue_server_config;
+typedef struct {
+  int64_t submit_ns; /* clue_now_ns() clock */
+  int64_t launch_ns;
+  int64_t done_ns;
+  int32_t batch_events;
+  int32_t reserved;
+  int64_t batch_points;
+} clue_server_timing;
+typedef struct {
+  int64_t events_submitted;
+  int64_t events_done;
+  int64_t points_done;
+  int64_t batches;
+  int64_t lane_points; /* per-launch point cap in force */
+} clue_server_stats;
+
+clue_status clue_server_create(const clue_server_config* cfg, clue_server** out);
+clue_status clue_server_destroy(clue_server* server); /* finishes queued work first */
+/* host_data: (ndim+1) x n SoA, copied before return (blocks while the staging ring
+ * is full). CLUE_ERR_CAPACITY when the event exceeds the per-launch cap. */
+clue_status clue_server_submit(clue_server* server, const clue_params* params, int32_t n,
+                               const float* host_data, uint64_t* id);
+/* Waits for an event (timeout_us < 0: no limit), copies its n labels into
+ * host_labels (NULL allowed) and releases it. */
+clue_status clue_server_wait(clue_server* server, uint64_t id, int32_t* host_labels,
+                             int64_t timeout_us, clue_server_timing* timing);
+clue_status clue_server_get_stats(clue_server* server, clue_server_stats* out);
+int64_t clue_now_ns(void); /* steady clock of the timings, in nanoseconds */
+
 /* Device memory helpers for callers without their own allocator. */
 clue_status clue_malloc(clue_ctx* ctx, size_t bytes, void** ptr);
 clue_status clue_free(clue_ctx* ctx, void* ptr);

@@ -148,9 +148,16 @@ def test_edge_cases(gpu_ctx, oracle_mod):
 
 
 def test_batched_with_empty_and_tiny_events(gpu_ctx, oracle_mod):
+    from cluestering_amd.device import Clusterer, PointsDevice
     c, w = read_points("data_1024.csv")
     sizes = np.array([0, 300, 1, 0, 2, 721], np.uint32)
-    _check(Case(c, w, 1.3, 10.0, 1.3, sizes=sizes, name="ragged"), gpu_ctx, oracle_mod)
+    _, ref = _check(Case(c, w, 1.3, 10.0, 1.3, sizes=sizes, name="ragged"), gpu_ctx, oracle_mod)
+    # getSampleAssociations: the event of every cluster, in the caller's numbering.
+    pts = PointsDevice.from_host(gpu_ctx, c, w)
+    eoc = gpu_ctx.empty(w.size, np.int32)
+    k = Clusterer(gpu_ctx, 1.3, 10.0, 1.3).make_clusters_batched(pts, sizes, event_of_cluster=eoc)
+    assert k == ref.n_seeds
+    np.testing.assert_array_equal(eoc.to_host()[:k], ref.event_of_cluster)
 
 
 def test_batched_reference_fixture(gpu_ctx, oracle_mod):
