@@ -12,7 +12,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_PKG, "libclue_amd.so")
+# The in-tree build; CLUE_AMD_LIBRARY selects another build of the same library.
+LIB_PATH = os.environ.get("CLUE_AMD_LIBRARY") or os.path.join(_PKG, "libclue_amd.so")
 
 MAX_DIM = 10
 
@@ -129,6 +130,9 @@ _SIGNATURES = {
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "clue_get_seeds": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "clue_get_n_clusters": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "clue_make_association": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                          C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)]),
     "clue_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "clue_stage_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_float),
                                      C.c_int32, C.POINTER(C.c_int32)]),

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "clue_amd.h"
+#include "clue_assoc.h"
 #include "clue_internal.h"
 #include "clue_runtime.h"
 
@@ -36,6 +37,7 @@ struct clue_ctx {
   hipEvent_t ev[kNumStages + 1] = {};
   bool ev_ready = false;
   bool ev_recorded[kNumStages + 1] = {};
+  Workspace aux;  // association maps (its own buffers: a pending run keeps ws)
 };
 
 struct clue_plan {
@@ -278,6 +280,45 @@ clue_status clue_get_seeds(clue_ctx* ctx, int32_t* seeds, int64_t capacity, int6
   if (!seeds || capacity < k) return fail(CLUE_ERR_CAPACITY, "seed buffer too small");
   CLUE_HIP(hipMemcpyAsync(seeds, ctx->last_seeds, sizeof(int32_t) * k, hipMemcpyDeviceToHost,
                           ctx->stream));
+  CLUE_HIP(hipStreamSynchronize(ctx->stream));
+  return CLUE_OK;
+}
+
+clue_status clue_make_association(clue_ctx* ctx, const int32_t* keys, int32_t n, int32_t* offsets,
+                                  int32_t offsets_capacity, int32_t* members, int32_t* n_bins,
+                                  int32_t* n_members) {
+  if (!ctx || !offsets || !n_bins || !n_members || (n > 0 && !keys))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
+  if (n < 0) return fail(CLUE_ERR_INVALID_ARGUMENT, "negative size");
+  ScopedDevice dev(ctx->device);
+  Workspace& w = ctx->aux;
+  int64_t hist = 0, scan = 0;
+  assoc_scratch_elems(n, &hist, &scan);
+  CLUE_HIP(w.buf[S_K0].reserve(sizeof(uint32_t) * n + 4));
+  CLUE_HIP(w.buf[S_V0].reserve(sizeof(int32_t) * n + 4));
+  CLUE_HIP(w.buf[S_K1].reserve(sizeof(uint32_t) * n + 4));
+  CLUE_HIP(w.buf[S_V1].reserve(sizeof(int32_t) * n + 4));
+  CLUE_HIP(w.buf[S_HIST].reserve(sizeof(int32_t) * hist));
+  CLUE_HIP(w.buf[S_SCAN].reserve(sizeof(int32_t) * scan));
+  CLUE_HIP(w.buf[S_NSEEDS].reserve(sizeof(int32_t) * 4));
+  int32_t* stats = w.buf[S_NSEEDS].as<int32_t>();
+  CLUE_HIP(assoc_stats(keys, n, stats, ctx->stream));
+  int32_t hs[2] = {0, 0};
+  CLUE_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+  CLUE_HIP(hipStreamSynchronize(ctx->stream));
+  // bins = max + 1 (make_associator), never below 0.
+  const int64_t bins = std::max<int64_t>(0, static_cast<int64_t>(hs[0]) + 1);
+  if (bins >= INT32_MAX) return fail(CLUE_ERR_CAPACITY, "key too large for an offsets array");
+  *n_bins = static_cast<int32_t>(bins);
+  *n_members = hs[1];
+  if (bins + 1 > offsets_capacity)
+    return fail(CLUE_ERR_CAPACITY, "offsets buffer too small: need max key + 2 entries");
+  if (hs[1] > 0 && !members) return fail(CLUE_ERR_INVALID_ARGUMENT, "null members");
+  AssocScratch sc{w.buf[S_K0].as<uint32_t>(), w.buf[S_V0].as<int32_t>(),
+                  w.buf[S_K1].as<uint32_t>(), w.buf[S_V1].as<int32_t>(),
+                  w.buf[S_HIST].as<int32_t>(), w.buf[S_SCAN].as<int32_t>()};
+  CLUE_HIP(assoc_build(keys, hs[1] > 0 ? n : 0, static_cast<int>(bins), sc, offsets, members,
+                       hs[1], ctx->stream));
   CLUE_HIP(hipStreamSynchronize(ctx->stream));
   return CLUE_OK;
 }

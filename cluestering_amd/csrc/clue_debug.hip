@@ -15,7 +15,7 @@ __global__ void k_debug_math(int n, const float* __restrict__ a, const float* __
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float x = a[i];
-  if (out_sqrt) out_sqrt[i] = __builtin_sqrtf(x);
+  if (out_sqrt) out_sqrt[i] = clue_dev::sqrt_rn(x);
   if (out_exp) out_exp[i] = clue_dev::expf_cw(x);
   if (out_div) out_div[i] = x / b[i];
 }

@@ -87,6 +87,21 @@ __device__ __forceinline__ float expf_cw(float x) {
   return (xin != xin) ? xin : e;
 }
 
+// Correctly rounded square root. For s >= 2^-96 the hardware estimate r (within
+// 1 ulp) is settled by Tuckerman's rounding test on its neighbours r - ulp and
+// r + ulp, whose fma residuals s - r' * r carry the sign of the exact error
+// there (no denormal intermediates). Zero, tiny, negative and NaN inputs take
+// the IEEE library sqrt.
+__device__ __forceinline__ float sqrt_rn(float s) {
+  const float r = __builtin_amdgcn_sqrtf(s);
+  const float rm = __int_as_float(__float_as_int(r) - 1);
+  const float rp = __int_as_float(__float_as_int(r) + 1);
+  float q = (__builtin_fmaf(-rm, r, s) <= 0.0f) ? rm : r;
+  q = (__builtin_fmaf(-rp, r, s) > 0.0f) ? rp : q;
+  if (__builtin_expect(!(s >= 0x1p-96f), 0)) q = __builtin_sqrtf(s);
+  return q;
+}
+
 // Ordering key of (rho, tag) such that, for a query point i and a candidate j,
 // key(j, candidate) > key(i, query) exactly when j is "higher" than i in the
 // reference's sense (ClusteringKernels.hpp:180-182):

@@ -61,6 +61,7 @@ struct BatchBuffers {
   int32_t* icount;     // [n_tiles_total]
   int32_t* n_items;    // [1]
   Item* items;         // [max_items]
+  int32_t* qperm;      // [n_total] query order within tiles; NULL = sorted order
   float* srho;
   uint64_t* skey;      // (rho, tag) ordering key of every sorted point
   int32_t* snh;

@@ -63,7 +63,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_NUM
+  S_QPERM, S_NUM
 };
 
 struct Workspace {
@@ -78,6 +78,7 @@ struct BatchPlan {
   int64_t n_tiles_total = 0;
   int64_t max_items = 0;
   int key_bits = 0;
+  bool group_queries = true;  // order queries by box class within tiles
 };
 
 clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t n_events,

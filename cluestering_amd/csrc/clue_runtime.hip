@@ -220,6 +220,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
       {S_ICOUNT, sizeof(int32_t) * T},
       {S_NITEMS, sizeof(int32_t) * 4},
       {S_ITEMS, sizeof(Item) * bp.max_items},
+      {S_QPERM, bp.group_queries ? sizeof(int32_t) * n : 0},
       {S_SRHO, sizeof(float) * n},
       {S_SKEY, sizeof(uint64_t) * n},
       {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(float) * n : 0},
@@ -266,6 +267,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   b.icount = ws.buf[S_ICOUNT].as<int32_t>();
   b.n_items = ws.buf[S_NITEMS].as<int32_t>();
   b.items = ws.buf[S_ITEMS].as<Item>();
+  b.qperm = bp.group_queries ? ws.buf[S_QPERM].as<int32_t>() : nullptr;
   b.srho = ws.buf[S_SRHO].as<float>();
   b.skey = ws.buf[S_SKEY].as<uint64_t>();
   b.snh = ws.buf[S_SNH].as<int32_t>();

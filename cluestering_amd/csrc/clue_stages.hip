@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <type_traits>
 
 #include "clue_device.h"
 #include "clue_internal.h"
@@ -295,7 +296,7 @@ __device__ __forceinline__ float metric_value(const float (&xi)[D], const float 
 template <int M>
 __device__ __forceinline__ float finish_distance(float v) {
   if constexpr (MetricTraits<M>::kSqrt)
-    return __builtin_sqrtf(v);
+    return sqrt_rn(v);
   else
     return v;
 }
@@ -355,6 +356,120 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 }
 
 // ---------------------------------------------------------------------------
+// Query order within a tile. A wave walks the union of its lanes' search boxes,
+// so lanes whose boxes have the same shape waste fewer candidate slots. Each
+// point gets a class from its box relative to its tile (per dimension, for the
+// first three: whether the box reaches below and above the tile's bin), and the
+// points of every tile with more than one wave's worth are counting-sorted by
+// class; items then take 64 consecutive entries of that order. Results do not
+// depend on the order: a point's traversal is its own, whichever lane runs it.
+// One wave per tile; ranking by ballot over the class bits.
+// ---------------------------------------------------------------------------
+constexpr int kQueryClassDims = 3;
+
+template <int D>
+__device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[D],
+                                                const float (&si)[D], float R,
+                                                const int (&tb)[D]) {
+  int lo[D], hi[D];
+  search_box<D>(g, xi, si, R, lo, hi);
+  uint32_t c = 0;
+#pragma unroll
+  for (int d = 0; d < (D < kQueryClassDims ? D : kQueryClassDims); ++d)
+    c |= ((lo[d] < tb[d] ? 2u : 0u) | (hi[d] > tb[d] ? 1u : 0u)) << (2 * d);
+  return c;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_query_order(
+    const RunParams rp, const EventDesc* __restrict__ events, int n_events,
+    const float* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
+    const float* __restrict__ sx, int64_t cs, const float* __restrict__ ssig, float R,
+    int32_t* __restrict__ qperm) {
+  constexpr int kClassBits = 2 * (D < kQueryClassDims ? D : kQueryClassDims);
+  constexpr int kClasses = 1 << kClassBits;
+  __shared__ int32_t base[kWaves][kClasses];
+  const int w = threadIdx.x / kWave;
+  const int t = blockIdx.x * kWaves + w;
+  if (t >= n_tiles) return;
+  const int lane = lane_id();
+  const int b = toff[t], e = toff[t + 1];
+  if (e - b <= kWave) {
+    if (b + lane < e) qperm[b + lane] = b + lane;
+    return;
+  }
+  // Event of tile t: last event with t_base <= t.
+  int lo_e = 0, hi_e = n_events - 1;
+  while (lo_e < hi_e) {
+    const int mid = (lo_e + hi_e + 1) >> 1;
+    if (events[mid].t_base <= t)
+      lo_e = mid;
+    else
+      hi_e = mid - 1;
+  }
+  const EventDesc ev = events[lo_e];
+  const Geo g = event_geo(geo, rp, lo_e, ev.nper);
+  int tb[D];
+  {
+    int r = t - ev.t_base;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      tb[d] = r % ev.nper;
+      r /= ev.nper;
+    }
+  }
+  auto cls = [&](int s, bool valid) -> uint32_t {
+    float xi[D], si[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      xi[d] = valid ? sx[d * cs + s] : 0.0f;
+      si[d] = (valid && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+    }
+    return valid ? query_class<D>(g, xi, si, R, tb) : 0u;
+  };
+  // Pass 1: class counts.
+  for (int c = lane; c < kClasses; c += kWave) base[w][c] = 0;
+  __builtin_amdgcn_wave_barrier();
+  for (int s0 = b; s0 < e; s0 += kWave) {
+    const int s = s0 + lane;
+    const bool valid = s < e;
+    const uint32_t c = cls(s, valid);
+    const uint64_t peers = match_peers(c, kClassBits, valid);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // Exclusive scan of the counts (one class per lane and round).
+  int carry = b;
+  for (int c0 = 0; c0 < kClasses; c0 += kWave) {
+    const int c = c0 + lane;
+    const int v = c < kClasses ? base[w][c] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (c < kClasses) base[w][c] = carry + x - v;
+    carry += __shfl(x, kWave - 1, 64);
+  }
+  __builtin_amdgcn_wave_barrier();
+  // Pass 2: scatter in class order, rounds and lanes in sorted order.
+  for (int s0 = b; s0 < e; s0 += kWave) {
+    const int s = s0 + lane;
+    const bool valid = s < e;
+    const uint32_t c = cls(s, valid);
+    const uint64_t peers = match_peers(c, kClassBits, valid);
+    int pos = 0;
+    if (valid) pos = base[w][c] + __popcll(peers & lanemask_lt());
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    if (valid) qperm[pos] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Kernel values (ConvolutionalKernel.hpp:21-57) for a candidate within the radius.
 // K_GAUSS2 is the gaussian whose 2*sigma*sigma is a power of two: there division
 // by it and multiplication by its reciprocal are the same correctly rounded
@@ -402,15 +517,17 @@ __global__ __launch_bounds__(kBlock) void k_density(
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
     const float* __restrict__ sw, const float* __restrict__ sfw, const float* __restrict__ ssig,
-    const uint32_t* __restrict__ stag, float* __restrict__ srho, uint64_t* __restrict__ skey) {
+    const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
+    float* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
   const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
   const Geo g = event_geo(geo, rp, it.event, ev.nper);
-  const int s = it.begin + lane_id();
-  const bool active = s < it.end;
+  const int q = it.begin + lane_id();
+  const bool active = q < it.end;
+  const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
   float xi[D], si[D];
 #pragma unroll
@@ -429,31 +546,41 @@ __global__ __launch_bounds__(kBlock) void k_density(
   }
   const float thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
   float rho = 0.0f;
+  // The diagonal term K(i,i) * w_i = w_i; the flat kernel's off-diagonal weights
+  // come pre-multiplied (fw = f * w).
+  const float wself = active ? sw[s] : 0.0f;
+  const float* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
 
   // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
-  auto add = [&](float v, int j, float thr, bool own) {
+  // wj is the candidate's column value, loaded with its block of candidates.
+  auto add = [&](float v, int j, float wj, float thr, auto own_c) {
+    constexpr bool own = decltype(own_c)::value;
     if constexpr (K == K_FLAT) {
       // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
       // the reference's addition of a zero term.
-      const float next = rho + ((own && j == s) ? sw[j] : sfw[j]);
+      const float next = rho + ((own && j == s) ? wself : wj);
       rho = (v <= thr) ? next : rho;
     } else {
       if (v <= thr) {
         const float kv = (own && j == s) ? 1.0f : kernel_value<M, K>(v, rp);
-        rho += kv * sw[j];
+        rho += kv * wj;
       }
     }
   };
-  auto walk = [&](int j0, int j1, float thr, bool own) {
+  // own_c: std::true_type for the point's own tile, where the diagonal term lies.
+  auto walk = [&](int j0, int j1, float thr, auto own_c) {
     int j = j0;
     for (; j + U <= j1; j += U) {
-      float v[U];
+      float v[U], w[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
 #pragma unroll
-      for (int k = 0; k < U; ++k) add(v[k], j + k, thr, own);
+      for (int k = 0; k < U; ++k) w[k] = wcol[j + k];
+#pragma unroll
+      for (int k = 0; k < U; ++k) add(v[k], j + k, w[k], thr, own_c);
     }
-    for (; j < j1; ++j) add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, thr, own);
+    for (; j < j1; ++j)
+      add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c);
   };
 
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
@@ -461,9 +588,9 @@ __global__ __launch_bounds__(kBlock) void k_density(
     const int j1 = toff[t + 1];
     const float thr = in ? thr_in : -1.0f;
     if (t == it.tile)
-      walk(j0, j1, thr, true);
+      walk(j0, j1, thr, std::true_type{});
     else
-      walk(j0, j1, thr, false);
+      walk(j0, j1, thr, std::false_type{});
   });
   if (active) {
     srho[s] = rho;
@@ -499,15 +626,17 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
     const float* __restrict__ ssig, const float* __restrict__ su,
     const uint32_t* __restrict__ stag, const float* __restrict__ srho,
-    const uint64_t* __restrict__ skey, int32_t* __restrict__ snh, float* __restrict__ sdelta) {
+    const uint64_t* __restrict__ skey, const int32_t* __restrict__ qperm,
+    int32_t* __restrict__ snh, float* __restrict__ sdelta) {
   constexpr int U = Unroll<D>::U;
   const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
   const Geo g = event_geo(geo, rp, it.event, ev.nper);
-  const int s = it.begin + lane_id();
-  const bool active = s < it.end;
+  const int q = it.begin + lane_id();
+  const bool active = q < it.end;
+  const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
   float xi[D], si[D];
 #pragma unroll
@@ -540,10 +669,8 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
   float bound = eff_thr;  // prune_threshold(eff_thr, delta)
 
   // The reference's acceptance test, in candidate order.
-  auto consider = [&](float v, int j) {
+  auto consider = [&](float v, float rj, uint32_t tj, int j) {
     const float dist = finish_distance<M>(v);
-    const float rj = srho[j];
-    const uint32_t tj = stag[j];
     if (dist <= eff &&
         ((dist < delta) ||
          ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
@@ -572,14 +699,21 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
       }
       // Most blocks hold no candidate for any lane: one wave-wide test skips them.
       if (ballot(any) != 0ull) {
+        float rj[U];
+        uint32_t tj[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          rj[k] = srho[j + k];
+          tj[k] = stag[j + k];
+        }
 #pragma unroll
         for (int k = 0; k < U; ++k)
-          if (pass[k] && v[k] <= bound) consider(v[k], j + k);
+          if (pass[k] && v[k] <= bound) consider(v[k], rj[k], tj[k], j + k);
       }
     }
     for (; j < j1; ++j) {
       const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
-      if ((skey[j] > key_i) & (v <= bound)) consider(v, j);
+      if ((skey[j] > key_i) & (v <= bound)) consider(v, srho[j], stag[j], j);
     }
   });
   if (active) {
@@ -666,7 +800,7 @@ static void launch_density_dmk(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
                      b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
-                     b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.srho, b.skey);
+                     b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
 }
 
 template <int D, int M>
@@ -688,8 +822,8 @@ static void launch_nh_dm(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp,
                      b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
-                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.snh,
-                     b.sdelta);
+                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm,
+                     b.snh, b.sdelta);
 }
 
 template <int D>
@@ -715,6 +849,30 @@ static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
         launch_nh_dm<D, M_MAHALANOBIS>(L);
   }
 #undef CLUE_METRIC_CASE
+}
+
+template <int D>
+static void launch_query_order_d(const NeighbourLaunch& L, float R) {
+  const BatchBuffers& b = *L.bb;
+  const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
+  hipLaunchKernelGGL((k_query_order<D>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
+                     b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
+                     b.sorted.sigma[0], R, b.qperm);
+}
+
+static void launch_query_order(const NeighbourLaunch& L, float R) {
+  switch (L.rp->D) {
+    case 1: launch_query_order_d<1>(L, R); break;
+    case 2: launch_query_order_d<2>(L, R); break;
+    case 3: launch_query_order_d<3>(L, R); break;
+    case 4: launch_query_order_d<4>(L, R); break;
+    case 5: launch_query_order_d<5>(L, R); break;
+    case 6: launch_query_order_d<6>(L, R); break;
+    case 7: launch_query_order_d<7>(L, R); break;
+    case 8: launch_query_order_d<8>(L, R); break;
+    case 9: launch_query_order_d<9>(L, R); break;
+    default: launch_query_order_d<10>(L, R); break;
+  }
 }
 
 static void launch_neighbour(const NeighbourLaunch& L, bool density) {
@@ -785,8 +943,10 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
   mark(kStageDensity);
+  if (bb.qperm) launch_query_order(L, rp.dc);
   launch_neighbour(L, true);
   mark(kStageNearestHigher);
+  if (bb.qperm && rp.dm != rp.dc) launch_query_order(L, rp.dm);
   launch_neighbour(L, false);
 
   // (d), (e)

@@ -295,6 +295,51 @@ class Clusterer:
         return out[: cnt.value]
 
 
+class AssociationMap:
+    """Key -> members CSR built on the device (AssociationMap<Device>,
+    data_structures/internal/MakeAssociator.hpp:15-45): ``offsets`` (bins + 1)
+    and ``indexes`` (the members, ascending within a key) are device arrays."""
+
+    def __init__(self, ctx: Context, keys: DeviceArray, n: Optional[int] = None):
+        n = int(np.prod(keys.shape)) if n is None else int(n)
+        lib = _capi.load()
+        cap = n + 2
+        self.offsets = ctx.empty(cap, np.int32)
+        self.indexes = ctx.empty(max(n, 1), np.int32)
+        bins, members = C.c_int32(), C.c_int32()
+        st = lib.clue_make_association(ctx.handle, C.c_void_p(keys.ptr), n,
+                                       C.c_void_p(self.offsets.ptr), cap,
+                                       C.c_void_p(self.indexes.ptr), C.byref(bins),
+                                       C.byref(members))
+        if st == _capi.ERR_CAPACITY and bins.value + 1 > cap:   # keys beyond n: size and retry
+            cap = bins.value + 1
+            self.offsets = ctx.empty(cap, np.int32)
+            st = lib.clue_make_association(ctx.handle, C.c_void_p(keys.ptr), n,
+                                           C.c_void_p(self.offsets.ptr), cap,
+                                           C.c_void_p(self.indexes.ptr), C.byref(bins),
+                                           C.byref(members))
+        _capi.check(st)
+        self.n_bins = int(bins.value)
+        self.n_members = int(members.value)
+
+    def size(self) -> int:
+        return self.n_bins
+
+    def to_host(self):
+        """(offsets, indexes) as host arrays."""
+        off = self.offsets.to_host()[: self.n_bins + 1]
+        idx = self.indexes.to_host()[: self.n_members]
+        return off, idx
+
+
+def get_clusters(points: "PointsDevice") -> AssociationMap:
+    """get_clusters(queue, d_points) (utils/detail/get_clusters.hpp:44-48): the
+    cluster -> points map of clustered device points, built on the device."""
+    if not points.clustered:
+        raise RuntimeError("the points have not been clustered")
+    return AssociationMap(points.ctx, points.cluster_index, points.n)
+
+
 class Plan:
     """A batch captured once in a hipGraph and replayed with one launch."""
 

@@ -152,6 +152,22 @@ clue_status clue_get_seeds(clue_ctx* ctx, int32_t* seeds, int64_t capacity, int6
  * utils/detail/get_cluster_properties.hpp:10-15). Waits. */
 clue_status clue_get_n_clusters(clue_ctx* ctx, int64_t* n_clusters);
 
+/* Key -> members association map built on the device (CSR), the device-side
+ * make_associator (data_structures/internal/MakeAssociator.hpp:15-45):
+ *   bins      = max key + 1 (0 when no key is >= 0);
+ *   members   = indices i with keys[i] >= 0, grouped by key, ascending within a key;
+ *   offsets   = bins + 1 boundaries (offsets[0] = 0, offsets[bins] = *n_members).
+ * keys, offsets and members are DEVICE arrays: offsets of capacity
+ * offsets_capacity (>= bins + 1), members of capacity n. *n_bins and *n_members
+ * (host) receive the sizes, also when CLUE_ERR_CAPACITY reports that offsets is
+ * too small (then nothing is written: call again with bins + 1 entries). Waits.
+ * With keys = cluster_index it is getClusters / get_clusters for device points
+ * (core/detail/Clusterer.hpp:212-217, utils/detail/get_clusters.hpp:29-48); with
+ * keys = event_of_cluster it is getSampleAssociations (:231-241). */
+clue_status clue_make_association(clue_ctx* ctx, const int32_t* keys, int32_t n, int32_t* offsets,
+                                  int32_t offsets_capacity, int32_t* members, int32_t* n_bins,
+                                  int32_t* n_members);
+
 /* Per-stage device time of the last run in microseconds (hipEvents on the context
  * stream); enable with clue_set_profiling. names[i] is a static string. */
 clue_status clue_set_profiling(clue_ctx* ctx, int32_t enabled);

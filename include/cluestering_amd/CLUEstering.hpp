@@ -248,6 +248,62 @@ class AssociationMapHost {
   std::vector<int32_t> m_off, m_idx;
 };
 
+// ---- Device CSR of key -> members (AssociationMap<Device>, data_structures/AssociationMap.hpp),
+// built on the device by clue_make_association (internal/MakeAssociator.hpp:15-45). -----------
+class AssociationMapDevice {
+ public:
+  AssociationMapDevice() = default;
+  // keys: device array of n association keys (cluster or event ids; negative = none).
+  AssociationMapDevice(clue_ctx* ctx, const int32_t* keys, int32_t n) : m_ctx{ctx} {
+    void* mem = nullptr;
+    detail::check(clue_malloc(ctx, 4 * std::size_t(std::max(n, 1)), &mem));
+    m_idx.reset(static_cast<int32_t*>(mem));
+    m_idx.get_deleter().ctx = ctx;
+    int32_t cap = n + 2;
+    alloc_offsets(cap);
+    clue_status st = clue_make_association(ctx, keys, n, m_off.get(), cap, m_idx.get(), &m_bins,
+                                           &m_elems);
+    if (st == CLUE_ERR_CAPACITY && m_bins + 1 > cap) {  // keys beyond n: size and retry
+      cap = m_bins + 1;
+      alloc_offsets(cap);
+      st = clue_make_association(ctx, keys, n, m_off.get(), cap, m_idx.get(), &m_bins, &m_elems);
+    }
+    detail::check(st);
+  }
+  // Number of keys (max key + 1) and of associated elements.
+  std::size_t size() const { return static_cast<std::size_t>(m_bins); }
+  std::size_t n_elements() const { return static_cast<std::size_t>(m_elems); }
+  // Device spans: bins + 1 offsets, n_elements member indices.
+  std::span<const int32_t> offsets() const { return {m_off.get(), std::size_t(m_bins) + 1}; }
+  std::span<const int32_t> indexes() const { return {m_idx.get(), std::size_t(m_elems)}; }
+  std::size_t count(int32_t key) const {
+    if (key < 0 || key >= m_bins) throw std::out_of_range("key out of range");
+    int32_t b[2];
+    detail::check(clue_memcpy_d2h(m_ctx, b, m_off.get() + key, sizeof(b)));
+    return static_cast<std::size_t>(b[1] - b[0]);
+  }
+  AssociationMapHost to_host() const {
+    std::vector<int32_t> off(static_cast<std::size_t>(m_bins) + 1);
+    std::vector<int32_t> idx(static_cast<std::size_t>(m_elems));
+    detail::check(clue_memcpy_d2h(m_ctx, off.data(), m_off.get(), 4 * off.size()));
+    if (!idx.empty())
+      detail::check(clue_memcpy_d2h(m_ctx, idx.data(), m_idx.get(), 4 * idx.size()));
+    return AssociationMapHost(std::move(off), std::move(idx));
+  }
+
+ private:
+  void alloc_offsets(int32_t count) {
+    void* off = nullptr;
+    detail::check(clue_malloc(m_ctx, 4 * std::size_t(count), &off));
+    m_off.reset(static_cast<int32_t*>(off));
+    m_off.get_deleter().ctx = m_ctx;
+  }
+  clue_ctx* m_ctx = nullptr;
+  std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_off{nullptr, {nullptr}};
+  std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_idx{nullptr, {nullptr}};
+  int32_t m_bins = 0, m_elems = 0;
+};
+
 // ---- Points ------------------------------------------------------------------------------
 // Host SoA (detail/HostViewPartition.hpp:15-21): Ndim coordinate columns, weights, and
 // the cluster indexes, in one owned buffer or in caller buffers.
@@ -503,6 +559,11 @@ template <std::size_t Ndim>
 AssociationMapHost get_clusters(const PointsHost<Ndim, float>& h) {
   return get_clusters(h.clusterIndexes());
 }
+// get_clusters(queue, d_points) (utils/detail/get_clusters.hpp:44-48): built on the device.
+template <std::size_t Ndim>
+AssociationMapDevice get_clusters(Queue& q, const PointsDevice<Ndim, float>& d) {
+  return AssociationMapDevice(q.ctx(), d.clusterIndexes().data(), d.size());
+}
 
 // ---- Clusterer (core/Clusterer.hpp:37-312) -----------------------------------------------
 template <std::size_t Ndim, class DataType = float>
@@ -582,6 +643,7 @@ class Clusterer {
     detail::check(clue_make_clusters(q.ctx(), &p, &pts, &res));
     m_last = q;
     m_event_of_cluster.clear();
+    m_eoc_dev.reset();
     d.mark_clustered();
   }
   template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
@@ -603,16 +665,17 @@ class Clusterer {
     clue_results res = d.c_results();
     void* eoc = nullptr;
     detail::check(clue_malloc(q.ctx(), 4 * std::size_t(d.size()), &eoc));
+    m_eoc_dev.reset(static_cast<int32_t*>(eoc));
+    m_eoc_dev.get_deleter().ctx = q.ctx();
     int64_t nclusters = 0;
     clue_status st = clue_make_clusters_batched(q.ctx(), &p, &pts, sizes.data(),
                                                 static_cast<int32_t>(sizes.size()), &res,
-                                                static_cast<int32_t*>(eoc), &nclusters);
+                                                m_eoc_dev.get(), &nclusters);
     if (st == CLUE_OK) {
       m_event_of_cluster.assign(std::size_t(nclusters), 0);
       if (nclusters > 0)
         st = clue_memcpy_d2h(q.ctx(), m_event_of_cluster.data(), eoc, 4 * std::size_t(nclusters));
     }
-    clue_free(q.ctx(), eoc);
     detail::check(st);
     m_last = q;
     d.mark_clustered();
@@ -627,13 +690,21 @@ class Clusterer {
     detail::check(clue_get_seeds(m_last->ctx(), s.data(), n, &n));
     return s;
   }
+  // getClusters (core/detail/Clusterer.hpp:205-217): cluster -> points, on the host
+  // for host points and on the device for device points.
   AssociationMapHost getClusters(const PointsHost<Ndim, float>& h) const { return get_clusters(h); }
-  // getSampleAssociations (core/detail/Clusterer.hpp:221-241): event -> clusters.
+  AssociationMapDevice getClusters(Queue& q, const PointsDevice<Ndim, float>& d) const {
+    return get_clusters(q, d);
+  }
+  // getSampleAssociations (core/detail/Clusterer.hpp:219-241): event -> clusters of the
+  // last batched run, on the host or on the device.
   AssociationMapHost getSampleAssociations(Queue&, const PointsHost<Ndim, float>&) const {
     return AssociationMapHost::from_associations(m_event_of_cluster);
   }
-  AssociationMapHost getSampleAssociations(Queue&, const PointsDevice<Ndim, float>&) const {
-    return AssociationMapHost::from_associations(m_event_of_cluster);
+  AssociationMapDevice getSampleAssociations(Queue& q, const PointsDevice<Ndim, float>&) const {
+    if (!m_eoc_dev) throw std::runtime_error("No batched run to associate. Run make_clusters first.");
+    return AssociationMapDevice(q.ctx(), m_eoc_dev.get(),
+                                static_cast<int32_t>(m_event_of_cluster.size()));
   }
 
  private:
@@ -656,6 +727,7 @@ class Clusterer {
   float m_dc{}, m_rhoc{}, m_dm{}, m_seed_dc{};
   std::array<uint8_t, Ndim> m_wrapped{};
   std::vector<int32_t> m_event_of_cluster;
+  std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_eoc_dev{nullptr, {nullptr}};
 };
 
 // read_csv (utils/detail/read_csv.hpp:22-104): header line, then Ndim coordinates and

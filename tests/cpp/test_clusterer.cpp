@@ -143,6 +143,13 @@ static void test_batched() {
   CHECK(same);
   auto seeds = algo.getSeeds();
   CHECK(seeds.size() == truth * 10);
+  // The device event -> clusters map equals the host one.
+  auto dassoc = algo.getSampleAssociations(q, d);
+  CHECK(dassoc.size() == 10);
+  CHECK(dassoc.n_elements() == truth * 10);
+  auto dh = dassoc.to_host();
+  CHECK(dh.offsets() == assoc.offsets());
+  CHECK(dh.indexes() == assoc.indexes());
 }
 
 static void test_device_points_and_clusters() {
@@ -161,6 +168,17 @@ static void test_device_points_and_clusters() {
   for (int v : h.clusterIndexes()) clustered += v >= 0;
   CHECK(total == clustered);
   CHECK(h.n_clusters() == 20);
+  // getClusters on device points: the same CSR, built on the device
+  // (members ascending within a cluster, as the host fill).
+  auto dclusters = algo.getClusters(q, d);
+  CHECK(dclusters.size() == clusters.size());
+  CHECK(dclusters.n_elements() == clustered);
+  auto back = dclusters.to_host();
+  CHECK(back.offsets() == clusters.offsets());
+  CHECK(back.indexes() == clusters.indexes());
+  CHECK(dclusters.count(3) == clusters.count(3));
+  auto free_fn = clue::get_clusters(q, d);
+  CHECK(free_fn.to_host().indexes() == clusters.indexes());
 }
 
 int main(int argc, char** argv) {

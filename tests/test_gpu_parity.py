@@ -44,6 +44,33 @@ def test_device_math_bits(gpu_ctx, oracle_mod):
     np.testing.assert_array_equal(e.view(np.uint32), oracle_mod.expf_array(x).view(np.uint32))
 
 
+def _bits_range(lo_bits, hi_bits):
+    return np.arange(lo_bits, hi_bits, dtype=np.uint32).view(np.float32)
+
+
+def test_device_sqrt_correctly_rounded(gpu_ctx):
+    """The stages' square root (hardware estimate + rounding test) equals the
+    correctly rounded IEEE sqrt. Every float in [1, 4) covers both exponent
+    parities of every mantissa; every float in [2^-97, 2^-94) and [2^126, +inf]
+    covers the ends of the fast range; then specials and a random sample."""
+    ranges = [(0x3F800000, 0x40800000),   # [1, 4)
+              (0x0F000000, 0x10800000),   # [2^-97, 2^-94): the switch at 2^-96
+              (0x7E800000, 0x7F800001)]   # [2^126, +inf]
+    for lo, hi in ranges:
+        a = _bits_range(lo, hi)
+        s, _, _ = debug_math(gpu_ctx, a, np.ones_like(a))
+        bad = np.nonzero(s.view(np.uint32) != np.sqrt(a).view(np.uint32))[0]
+        assert bad.size == 0, f"range {lo:#x}: first mismatch at {a[bad[0]]!r}"
+    rng = np.random.default_rng(8)
+    a = np.concatenate([rng.integers(0, 0x7F800000, 1 << 22, dtype=np.uint32).view(np.float32),
+                        np.array([0.0, -0.0, 1e-45, 1e-40, 1.17549435e-38, np.inf], np.float32)])
+    s, _, _ = debug_math(gpu_ctx, a, np.ones_like(a))
+    np.testing.assert_array_equal(s.view(np.uint32), np.sqrt(a).view(np.uint32))
+    neg = np.array([-1.0, -1e-30, -np.inf, np.nan], np.float32)
+    s, _, _ = debug_math(gpu_ctx, neg, np.ones_like(neg))
+    assert np.isnan(s).all()
+
+
 def test_blobs2d_16k(gpu_ctx, oracle_mod):
     _check(_workload_case(synthetic.blobs2d_16k(), "blobs2d_16k"), gpu_ctx, oracle_mod)
 
