@@ -236,7 +236,16 @@ class clusterer:
                         self._seeding_distance, wrapped, data.coords, data.results, self._kernel,
                         data.n_dim, batch_sample_sizes, data.n_points, device_id, self._metric)
         finish = time.time_ns()
-        cluster_ids = data.results
+        self._set_results(data.results)
+        self._elapsed_time = (finish - start) / 1e6
+        if verbose:
+            print(f"CLUE executed in {self._elapsed_time} ms")
+            print(f"Number of clusters found: {self.clust_prop.n_clusters}")
+
+    def _set_results(self, cluster_ids: np.ndarray) -> None:
+        """cluster_properties of a label array: n_clusters = max + 1, the member lists
+        in ascending point index, and the sizes."""
+        cluster_ids = np.asarray(cluster_ids)
         n_clusters = int(np.max(cluster_ids)) + 1 if cluster_ids.size else 0
         valid = np.nonzero(cluster_ids >= 0)[0]
         order = valid[np.argsort(cluster_ids[valid], kind="stable")]
@@ -252,10 +261,40 @@ class clusterer:
             output_df = None
         self.clust_prop = cluster_properties(n_clusters, cluster_ids, cluster_points,
                                              np.asarray(counts), output_df)
-        self._elapsed_time = (finish - start) / 1e6
-        if verbose:
-            print(f"CLUE executed in {self._elapsed_time} ms")
-            print(f"Number of clusters found: {self.clust_prop.n_clusters}")
+
+    # -- output (CLUEstering.py:1155-1215) --------------------------------------------------
+    def to_csv(self, output_folder: str, file_name: str) -> None:
+        """Writes the coordinates, the weight and the cluster id of every point
+        (columns x0..x{n-1}, weight, cluster_ids)."""
+        import csv
+        import os
+        if not file_name.endswith(".csv"):
+            file_name += ".csv"
+        path = os.path.join(output_folder, file_name)
+        d = self.clust_data
+        cols = [d.coords[k] for k in range(d.n_dim)] + [d.coords[-1], self.clust_prop.cluster_ids]
+        with open(path, "w", newline="") as f:
+            wr = csv.writer(f)
+            wr.writerow([f"x{k}" for k in range(d.n_dim)] + ["weight", "cluster_ids"])
+            for i in range(d.n_points):
+                wr.writerow([repr(float(c[i])) for c in cols[:-1]] + [int(cols[-1][i])])
+
+    def import_clusterer(self, input_folder: str, file_name: str) -> None:
+        """Reads back a file written by to_csv: the points and their cluster ids."""
+        import os
+        path = os.path.join(input_folder, file_name)
+        if not os.path.exists(path):
+            raise ValueError(f"The file {path} does not exist.")
+        with open(path) as f:
+            header = f.readline().strip().split(",")
+        if len(header) < 3 or header[-1] != "cluster_ids" or header[-2] != "weight":
+            raise ValueError(f"{path} is not a clusterer output (x..., weight, cluster_ids).")
+        table = np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2, dtype=np.float64)
+        ndim = len(header) - 2
+        coords = np.ascontiguousarray(table[:, :ndim + 1].T.astype(np.float32))
+        ids = table[:, -1].astype(np.int32)
+        self.clust_data = ClusteringDataSoA(coords, ids.copy(), ndim, coords.shape[1])
+        self._set_results(ids)
 
     def fit(self, data, backend: str = "gpu hip", batch_sample_sizes=None, device_id: int = 0,
             verbose: bool = False, dimensions=None) -> "clusterer":

@@ -75,6 +75,14 @@ class Results(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("group_queries", C.c_int32),
+        ("xcd_chunk_log2", C.c_int32),
+        ("reserved", C.c_int32 * 6),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("device", C.c_int32),
@@ -133,6 +141,9 @@ _SIGNATURES = {
     "clue_make_association": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                           C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_int32)]),
+    "clue_set_tuning": (C.c_int32, [C.c_void_p, C.c_void_p]),
+    "clue_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.c_int32,
+                                    C.POINTER(C.c_uint32), C.c_int32, C.POINTER(Tuning)]),
     "clue_set_profiling": (C.c_int32, [C.c_void_p, C.c_int32]),
     "clue_stage_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_float),
                                      C.c_int32, C.POINTER(C.c_int32)]),
@@ -149,6 +160,7 @@ _SIGNATURES = {
     "clue_debug_math": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "clue_debug_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "clue_debug_poison": (C.c_int32, [C.c_void_p]),
     "clue_server_create": (C.c_int32, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "clue_server_destroy": (C.c_int32, [C.c_void_p]),
     "clue_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p,

@@ -38,6 +38,8 @@ struct clue_ctx {
   bool ev_ready = false;
   bool ev_recorded[kNumStages + 1] = {};
   Workspace aux;  // association maps (its own buffers: a pending run keeps ws)
+  bool tuned = false;  // clue_set_tuning fixed `tuning`
+  clue_tuning tuning{};
 };
 
 struct clue_plan {
@@ -82,7 +84,7 @@ clue_status run_batch(clue_ctx* ctx, const clue_params* p, const clue_points* pt
     ctx->pending = false;
   }
   BatchPlan& bp = ctx->plan;
-  st = make_batch_plan(p, sizes, n_events, &bp);
+  st = make_batch_plan(p, sizes, n_events, &bp, ctx->tuned ? &ctx->tuning : nullptr);
   if (st) return st;
   st = check_points(p, pts, bp.n_total);
   if (st) return st;
@@ -91,7 +93,7 @@ clue_status run_batch(clue_ctx* ctx, const clue_params* p, const clue_points* pt
   st = prepare_batch(ctx->ws, bp, p, pts, res, ctx->stream, &bb, &ra);
   if (st) return st;
   ra.event_of_cluster = event_of_cluster;
-  const RunParams rp = make_run_params(p, pts, static_cast<int>(bp.events.size()), bp.n_total);
+  const RunParams rp = make_run_params(p, pts, bp);
   StageHook hook{mark_stage, ctx};
   if (ctx->profiling && !ctx->ev_ready) {
     for (auto& e : ctx->ev) CLUE_HIP(hipEventCreate(&e));
@@ -323,6 +325,51 @@ clue_status clue_make_association(clue_ctx* ctx, const int32_t* keys, int32_t n,
   return CLUE_OK;
 }
 
+clue_status clue_debug_poison(clue_ctx* ctx) {
+  if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
+  ScopedDevice dev(ctx->device);
+  CLUE_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->pending = false;
+  for (DevBuf& b : ctx->ws.buf)
+    if (b.p) CLUE_HIP(hipMemsetAsync(b.p, 0xFF, b.cap, ctx->stream));
+  CLUE_HIP(hipStreamSynchronize(ctx->stream));
+  return CLUE_OK;
+}
+
+clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning) {
+  if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
+  if (tuning && (tuning->xcd_chunk_log2 < -1 || tuning->xcd_chunk_log2 > 12))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "xcd_chunk_log2 must lie in [-1, 12]");
+  ctx->tuned = tuning != nullptr;
+  if (tuning) {
+    ctx->tuning = clue_tuning{};
+    ctx->tuning.group_queries = tuning->group_queries ? 1 : 0;
+    ctx->tuning.xcd_chunk_log2 = tuning->xcd_chunk_log2;
+  }
+  return CLUE_OK;
+}
+
+clue_status clue_get_tuning(clue_ctx* ctx, const clue_params* params, int32_t n,
+                            const uint32_t* sizes, int32_t n_events, clue_tuning* out) {
+  if (!ctx || !params || !out) return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
+  if (ctx->tuned) {
+    *out = ctx->tuning;
+    return CLUE_OK;
+  }
+  int64_t total = 0, events = 0;
+  if (sizes) {
+    for (int32_t e = 0; e < n_events; ++e) {
+      total += sizes[e];
+      events += sizes[e] > 0 ? 1 : 0;
+    }
+  } else {
+    total = n;
+    events = 1;
+  }
+  *out = table_tuning(params->ndim, total, events);
+  return CLUE_OK;
+}
+
 clue_status clue_set_profiling(clue_ctx* ctx, int32_t enabled) {
   if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
   ctx->profiling = enabled != 0;
@@ -349,7 +396,7 @@ clue_status clue_plan_create(clue_ctx* ctx, const clue_params* params, const clu
     return fail(CLUE_ERR_INVALID_ARGUMENT, "cluster_index is required");
   if (!sizes || n_events < 1) return fail(CLUE_ERR_INVALID_ARGUMENT, "no events");
   BatchPlan bp;
-  st = make_batch_plan(params, sizes, n_events, &bp);
+  st = make_batch_plan(params, sizes, n_events, &bp, ctx->tuned ? &ctx->tuning : nullptr);
   if (st) return st;
   st = check_points(params, points, bp.n_total);
   if (st) return st;
@@ -362,8 +409,7 @@ clue_status clue_plan_create(clue_ctx* ctx, const clue_params* params, const clu
   if (st) return st;
   CLUE_HIP(hipStreamSynchronize(ctx->stream));  // descriptors uploaded before capture
   ra.event_of_cluster = event_of_cluster;
-  const RunParams rp =
-      make_run_params(params, points, static_cast<int>(bp.events.size()), bp.n_total);
+  const RunParams rp = make_run_params(params, points, bp);
   CLUE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
   launch_pipeline(rp, point_arrays(params, points), bb, ra, ctx->stream, nullptr);
   hipError_t le = hipGetLastError();

@@ -61,7 +61,9 @@ struct BatchBuffers {
   int32_t* icount;     // [n_tiles_total]
   int32_t* n_items;    // [1]
   Item* items;         // [max_items]
-  int32_t* qperm;      // [n_total] query order within tiles; NULL = sorted order
+  int32_t* qperm;      // [n_total] density query order within tiles; NULL = sorted order
+  int32_t* qperm_nh;   // [n_total] nearest-higher query order (may alias qperm)
+  int32_t* qfine;      // [n_total] Morton order within tiles (input of both)
   float* srho;
   uint64_t* skey;      // (rho, tag) ordering key of every sorted point
   int32_t* snh;

@@ -63,7 +63,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_NUM
 };
 
 struct Workspace {
@@ -78,16 +78,18 @@ struct BatchPlan {
   int64_t n_tiles_total = 0;
   int64_t max_items = 0;
   int key_bits = 0;
-  bool group_queries = true;  // order queries by box class within tiles
+  clue_tuning tuning{};
 };
 
+// Tuning of a batch: `fixed` when given, else the built-in table's entry for the
+// batch's ndim and mean event size.
+clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events);
 clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t n_events,
-                            BatchPlan* bp);
+                            BatchPlan* bp, const clue_tuning* fixed = nullptr);
 // Largest float t with sqrtf(t) <= r.
 float sq_max(float r);
 clue_status validate(const clue_params* p);
-RunParams make_run_params(const clue_params* p, const clue_points* pts, int n_events,
-                          int64_t n_total);
+RunParams make_run_params(const clue_params* p, const clue_points* pts, const BatchPlan& bp);
 clue_status check_points(const clue_params* p, const clue_points* pts, int64_t n_expected);
 PointArrays point_arrays(const clue_params* p, const clue_points* pts);
 // Reserves the workspace of one batch, uploads its descriptors on stream s and

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "clue_runtime.h"
+#include "clue_tuning_table.h"
 
 namespace clue_rt {
 
@@ -22,8 +23,23 @@ clue_status fail(clue_status code, const std::string& msg) {
 
 const char* last_error() { return g_last_error.c_str(); }
 
+clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events) {
+  const int64_t mean = n_events > 0 ? n_total / n_events : n_total;
+  for (const TuningEntry& t : kTuningTable)
+    if ((t.ndim == 0 || t.ndim == ndim) && mean <= t.max_mean_event) {
+      clue_tuning out{};
+      out.group_queries = t.group_queries;
+      out.xcd_chunk_log2 = t.xcd_chunk_log2;
+      return out;
+    }
+  clue_tuning out{};
+  out.group_queries = 1;
+  out.xcd_chunk_log2 = 4;
+  return out;
+}
+
 clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t n_events,
-                            BatchPlan* bp) {
+                            BatchPlan* bp, const clue_tuning* fixed) {
   const int D = p->ndim;
   const int ppt = p->points_per_tile > 0 ? p->points_per_tile : kDefaultPointsPerTile;
   bp->events.clear();
@@ -65,6 +81,8 @@ clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t
   }
   bp->n_total = off;
   bp->n_tiles_total = tiles;
+  bp->tuning = fixed ? *fixed
+                     : table_tuning(D, off, static_cast<int64_t>(bp->events.size()));
   bp->max_items = (off + 63) / 64 + tiles;
   int bits = 0;
   while ((int64_t(1) << bits) < tiles) ++bits;
@@ -118,9 +136,11 @@ clue_status validate(const clue_params* p) {
   return CLUE_OK;
 }
 
-RunParams make_run_params(const clue_params* p, const clue_points* pts, int n_events,
-                          int64_t n_total) {
+RunParams make_run_params(const clue_params* p, const clue_points* pts, const BatchPlan& bp) {
+  const int n_events = static_cast<int>(bp.events.size());
+  const int64_t n_total = bp.n_total;
   RunParams rp{};
+  rp.xcd_chunk_log2 = bp.tuning.xcd_chunk_log2;
   rp.D = p->ndim;
   rp.kernel = p->kernel;
   rp.metric = p->metric;
@@ -220,7 +240,9 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
       {S_ICOUNT, sizeof(int32_t) * T},
       {S_NITEMS, sizeof(int32_t) * 4},
       {S_ITEMS, sizeof(Item) * bp.max_items},
-      {S_QPERM, bp.group_queries ? sizeof(int32_t) * n : 0},
+      {S_QPERM, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
+      {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc) ? sizeof(int32_t) * n : 0},
+      {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_SRHO, sizeof(float) * n},
       {S_SKEY, sizeof(uint64_t) * n},
       {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(float) * n : 0},
@@ -267,7 +289,10 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   b.icount = ws.buf[S_ICOUNT].as<int32_t>();
   b.n_items = ws.buf[S_NITEMS].as<int32_t>();
   b.items = ws.buf[S_ITEMS].as<Item>();
-  b.qperm = bp.group_queries ? ws.buf[S_QPERM].as<int32_t>() : nullptr;
+  b.qperm = bp.tuning.group_queries ? ws.buf[S_QPERM].as<int32_t>() : nullptr;
+  b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc) ? ws.buf[S_QPERM_NH].as<int32_t>()
+                                                          : b.qperm;
+  b.qfine = bp.tuning.group_queries ? ws.buf[S_QFINE].as<int32_t>() : nullptr;
   b.srho = ws.buf[S_SRHO].as<float>();
   b.skey = ws.buf[S_SKEY].as<uint64_t>();
   b.snh = ws.buf[S_SNH].as<int32_t>();

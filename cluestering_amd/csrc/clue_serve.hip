@@ -211,7 +211,7 @@ struct clue_server {
     ra.local_ids = 1;
     hipLaunchKernelGGL(k_repack, dim3(bb.n_chunks), dim3(256), 0, L.stream,
                        L.reqdesc.as<ReqDesc>(), bb.chunks, staging, cols, D, n_total);
-    const RunParams rp = make_run_params(&p, &pts, static_cast<int>(batch.size()), n_total);
+    const RunParams rp = make_run_params(&p, &pts, L.plan);
     launch_pipeline(rp, point_arrays(&p, &pts), bb, ra, L.stream, nullptr);
     CLUE_HIP(hipGetLastError());
     CLUE_HIP(hipMemcpyAsync(L.labels_host, L.labels.p, sizeof(int32_t) * n_total,

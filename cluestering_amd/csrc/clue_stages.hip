@@ -32,13 +32,20 @@ enum { M_EUCLID = 0, M_WEUCLID = 1, M_PERIODIC = 2, M_MANHATTAN = 3, M_CHEBYSHEV
        M_WCHEBYSHEV = 5, M_MAHALANOBIS = 6 };
 enum { K_FLAT = 0, K_EXP = 1, K_GAUSS = 2 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Remapping block b to (b mod 8) * (nb / 8) + b / 8 gives every XCD a contiguous
-// range of work items (neighbouring tiles of the same events), so tile data is
-// shared in one L2 instead of being fetched by all eight. nb is a multiple of 8.
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b mod 8), each with its own L2. The neighbour stages deal chunks of G = 2^k
+// consecutive work-item blocks to the XCDs in turn: an XCD's L2 then sees runs
+// of neighbouring tiles, while the work stays spread evenly over the XCDs (a
+// contiguous eighth per XCD leaves them unequal, since events differ in cost).
+// Blocks past the last whole round of chunks keep their index.
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  return (b % kXcds) * (nb / kXcds) + b / kXcds;
+__device__ __forceinline__ int xcd_block(int b, int nb, int log2_chunk) {
+  if (log2_chunk < 0) return b;
+  const int round = kXcds << log2_chunk;
+  if (b >= nb / round * round) return b;
+  const int xcd = b % kXcds;
+  const int k = b / kXcds;  // b is the k-th block on its XCD
+  return ((((k >> log2_chunk) * kXcds) + xcd) << log2_chunk) + (k & ((1 << log2_chunk) - 1));
 }
 
 template <int M>
@@ -356,16 +363,21 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 }
 
 // ---------------------------------------------------------------------------
-// Query order within a tile. A wave walks the union of its lanes' search boxes,
-// so lanes whose boxes have the same shape waste fewer candidate slots. Each
-// point gets a class from its box relative to its tile (per dimension, for the
-// first three: whether the box reaches below and above the tile's bin), and the
-// points of every tile with more than one wave's worth are counting-sorted by
-// class; items then take 64 consecutive entries of that order. Results do not
-// depend on the order: a point's traversal is its own, whichever lane runs it.
-// One wave per tile; ranking by ballot over the class bits.
+// Query order within a tile. A wave walks the union of its lanes' search boxes
+// and runs a candidate's kernel when any lane has it within the radius, so waves
+// of lanes with alike boxes and nearby points waste fewer slots. The points of
+// every tile with more than one wave's worth are ordered by
+//   (box class, Morton index of the point's sub-cell of the tile),
+// where the class says, per dimension (first three), whether the search box
+// reaches below and above the tile's bin; items then take 64 consecutive entries
+// of that order. Results do not depend on the order: a point's traversal is its
+// own, whichever lane runs it.
+// Two stable counting sorts per tile, one wave per tile, ranking by ballot:
+//   kOrderFine:  sorted order -> Morton order (shared by both stages);
+//   kOrderClass: Morton order -> (class, Morton) for the stage's radius.
 // ---------------------------------------------------------------------------
-constexpr int kQueryClassDims = 3;
+constexpr int kQueryDims = 3;  // dimensions that enter the keys
+enum { kOrderFine = 0, kOrderClass = 1 };
 
 template <int D>
 __device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[D],
@@ -375,27 +387,51 @@ __device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[
   search_box<D>(g, xi, si, R, lo, hi);
   uint32_t c = 0;
 #pragma unroll
-  for (int d = 0; d < (D < kQueryClassDims ? D : kQueryClassDims); ++d)
+  for (int d = 0; d < (D < kQueryDims ? D : kQueryDims); ++d)
     c |= ((lo[d] < tb[d] ? 2u : 0u) | (hi[d] > tb[d] ? 1u : 0u)) << (2 * d);
   return c;
 }
 
+// Morton index of the point's sub-cell: 8 bits over the first (up to) three
+// dimensions of the tile, bits interleaved from the most significant.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_query_order(
+__device__ __forceinline__ uint32_t query_fine(const Geo& g, const float (&xi)[D],
+                                               const int (&tb)[D]) {
+  constexpr int kD = D < kQueryDims ? D : kQueryDims;
+  constexpr int kB = 8 / kD;  // bits per dimension
+  constexpr int kS = 1 << kB;
+  uint32_t q[kD];
+#pragma unroll
+  for (int d = 0; d < kD; ++d) {
+    const float c = g.wrapped[d] ? normalize(g, xi[d], d) : xi[d];
+    const float f = ((c - g.mn[d]) / g.side[d] - static_cast<float>(tb[d])) * kS;
+    q[d] = (f >= 0.0f) ? (f < static_cast<float>(kS) ? static_cast<uint32_t>(f) : kS - 1u) : 0u;
+  }
+  uint32_t key = 0;
+#pragma unroll
+  for (int b = kB - 1; b >= 0; --b)
+#pragma unroll
+    for (int d = 0; d < kD; ++d) key = (key << 1) | ((q[d] >> b) & 1u);
+  return key;
+}
+
+template <int D, int Mode>
+__global__ __launch_bounds__(kBlock) void k_tile_order(
     const RunParams rp, const EventDesc* __restrict__ events, int n_events,
     const float* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
     const float* __restrict__ sx, int64_t cs, const float* __restrict__ ssig, float R,
-    int32_t* __restrict__ qperm) {
-  constexpr int kClassBits = 2 * (D < kQueryClassDims ? D : kQueryClassDims);
-  constexpr int kClasses = 1 << kClassBits;
-  __shared__ int32_t base[kWaves][kClasses];
+    const int32_t* __restrict__ in_order, int32_t* __restrict__ out_order) {
+  constexpr int kD = D < kQueryDims ? D : kQueryDims;
+  constexpr int kKeyBits = Mode == kOrderFine ? (8 / kD) * kD : 2 * kD;
+  constexpr int kKeys = 1 << kKeyBits;
+  __shared__ int32_t base[kWaves][kKeys];
   const int w = threadIdx.x / kWave;
   const int t = blockIdx.x * kWaves + w;
   if (t >= n_tiles) return;
   const int lane = lane_id();
   const int b = toff[t], e = toff[t + 1];
-  if (e - b <= kWave) {
-    if (b + lane < e) qperm[b + lane] = b + lane;
+  if (e - b <= kWave) {  // one item: order is irrelevant
+    if (b + lane < e) out_order[b + lane] = b + lane;
     return;
   }
   // Event of tile t: last event with t_base <= t.
@@ -418,54 +454,61 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       r /= ev.nper;
     }
   }
-  auto cls = [&](int s, bool valid) -> uint32_t {
+  // Point at position i of the input order, and its key.
+  auto point = [&](int i) { return in_order ? in_order[i] : i; };
+  auto key = [&](int s, bool valid) -> uint32_t {
+    if (!valid) return 0u;
     float xi[D], si[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      xi[d] = valid ? sx[d * cs + s] : 0.0f;
-      si[d] = (valid && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+      xi[d] = sx[d * cs + s];
+      si[d] = rp.has_sigma ? ssig[d * cs + s] : 0.0f;
     }
-    return valid ? query_class<D>(g, xi, si, R, tb) : 0u;
+    if constexpr (Mode == kOrderFine)
+      return query_fine<D>(g, xi, tb);
+    else
+      return query_class<D>(g, xi, si, R, tb);
   };
-  // Pass 1: class counts.
-  for (int c = lane; c < kClasses; c += kWave) base[w][c] = 0;
+  // Pass 1: key counts.
+  for (int c = lane; c < kKeys; c += kWave) base[w][c] = 0;
   __builtin_amdgcn_wave_barrier();
-  for (int s0 = b; s0 < e; s0 += kWave) {
-    const int s = s0 + lane;
-    const bool valid = s < e;
-    const uint32_t c = cls(s, valid);
-    const uint64_t peers = match_peers(c, kClassBits, valid);
+  for (int i0 = b; i0 < e; i0 += kWave) {
+    const int i = i0 + lane;
+    const bool valid = i < e;
+    const uint32_t c = key(valid ? point(i) : 0, valid);
+    const uint64_t peers = match_peers(c, kKeyBits, valid);
     __builtin_amdgcn_wave_barrier();
     if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
     __builtin_amdgcn_wave_barrier();
   }
-  // Exclusive scan of the counts (one class per lane and round).
+  // Exclusive scan of the counts, 64 keys per round.
   int carry = b;
-  for (int c0 = 0; c0 < kClasses; c0 += kWave) {
+  for (int c0 = 0; c0 < kKeys; c0 += kWave) {
     const int c = c0 + lane;
-    const int v = c < kClasses ? base[w][c] : 0;
+    const int v = c < kKeys ? base[w][c] : 0;
     int x = v;
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
       const int y = __shfl_up(x, off, 64);
       if (lane >= off) x += y;
     }
-    if (c < kClasses) base[w][c] = carry + x - v;
+    if (c < kKeys) base[w][c] = carry + x - v;
     carry += __shfl(x, kWave - 1, 64);
   }
   __builtin_amdgcn_wave_barrier();
-  // Pass 2: scatter in class order, rounds and lanes in sorted order.
-  for (int s0 = b; s0 < e; s0 += kWave) {
-    const int s = s0 + lane;
-    const bool valid = s < e;
-    const uint32_t c = cls(s, valid);
-    const uint64_t peers = match_peers(c, kClassBits, valid);
+  // Pass 2: scatter in key order; equal keys keep the input order.
+  for (int i0 = b; i0 < e; i0 += kWave) {
+    const int i = i0 + lane;
+    const bool valid = i < e;
+    const int s = valid ? point(i) : 0;
+    const uint32_t c = key(s, valid);
+    const uint64_t peers = match_peers(c, kKeyBits, valid);
     int pos = 0;
     if (valid) pos = base[w][c] + __popcll(peers & lanemask_lt());
     __builtin_amdgcn_wave_barrier();
     if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
     __builtin_amdgcn_wave_barrier();
-    if (valid) qperm[pos] = s;
+    if (valid) out_order[pos] = s;
   }
 }
 
@@ -520,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_density(
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
     float* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
-  const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -629,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ qperm,
     int32_t* __restrict__ snh, float* __restrict__ sdelta) {
   constexpr int U = Unroll<D>::U;
-  const int item = xcd_block(blockIdx.x, gridDim.x) * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * kWaves + threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -822,7 +865,7 @@ static void launch_nh_dm(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp,
                      b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
-                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm,
+                     b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm_nh,
                      b.snh, b.sdelta);
 }
 
@@ -852,26 +895,32 @@ static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
 }
 
 template <int D>
-static void launch_query_order_d(const NeighbourLaunch& L, float R) {
+static void launch_query_order_d(const NeighbourLaunch& L, int32_t* out, float R) {
   const BatchBuffers& b = *L.bb;
   const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((k_query_order<D>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
-                     b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
-                     b.sorted.sigma[0], R, b.qperm);
+  if (out == b.qfine)
+    hipLaunchKernelGGL((k_tile_order<D, kOrderFine>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp,
+                       b.events, b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
+                       b.sorted.sigma[0], R, nullptr, b.qfine);
+  else
+    hipLaunchKernelGGL((k_tile_order<D, kOrderClass>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp,
+                       b.events, b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
+                       b.sorted.sigma[0], R, b.qfine, out);
 }
 
-static void launch_query_order(const NeighbourLaunch& L, float R) {
+// out == qfine: the Morton order; else the (class, Morton) order for radius R.
+static void launch_query_order(const NeighbourLaunch& L, int32_t* out, float R) {
   switch (L.rp->D) {
-    case 1: launch_query_order_d<1>(L, R); break;
-    case 2: launch_query_order_d<2>(L, R); break;
-    case 3: launch_query_order_d<3>(L, R); break;
-    case 4: launch_query_order_d<4>(L, R); break;
-    case 5: launch_query_order_d<5>(L, R); break;
-    case 6: launch_query_order_d<6>(L, R); break;
-    case 7: launch_query_order_d<7>(L, R); break;
-    case 8: launch_query_order_d<8>(L, R); break;
-    case 9: launch_query_order_d<9>(L, R); break;
-    default: launch_query_order_d<10>(L, R); break;
+    case 1: launch_query_order_d<1>(L, out, R); break;
+    case 2: launch_query_order_d<2>(L, out, R); break;
+    case 3: launch_query_order_d<3>(L, out, R); break;
+    case 4: launch_query_order_d<4>(L, out, R); break;
+    case 5: launch_query_order_d<5>(L, out, R); break;
+    case 6: launch_query_order_d<6>(L, out, R); break;
+    case 7: launch_query_order_d<7>(L, out, R); break;
+    case 8: launch_query_order_d<8>(L, out, R); break;
+    case 9: launch_query_order_d<9>(L, out, R); break;
+    default: launch_query_order_d<10>(L, out, R); break;
   }
 }
 
@@ -934,7 +983,6 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   hipLaunchKernelGGL(k_item_fill, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items);
 
-  // (b), (c)
   NeighbourLaunch L;
   L.rp = &rp;
   L.bb = &bb;
@@ -942,11 +990,16 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   const int64_t blocks = (bb.max_items + kWaves - 1) / kWaves;
   L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
+  if (bb.qperm) {  // query orders of the two neighbour stages
+    launch_query_order(L, bb.qfine, 0.0f);
+    launch_query_order(L, bb.qperm, rp.dc);
+    if (bb.qperm_nh != bb.qperm) launch_query_order(L, bb.qperm_nh, rp.dm);
+  }
+
+  // (b), (c)
   mark(kStageDensity);
-  if (bb.qperm) launch_query_order(L, rp.dc);
   launch_neighbour(L, true);
   mark(kStageNearestHigher);
-  if (bb.qperm && rp.dm != rp.dc) launch_query_order(L, rp.dm);
   launch_neighbour(L, false);
 
   // (d), (e)

@@ -1,0 +1,28 @@
+// Built-in launch tuning, by ndim (0 = any) and mean event size (first match).
+// Regenerated from MI355X measurements by cluestering_amd/autotune.py.
+#pragma once
+
+#include <cstdint>
+
+namespace clue_rt {
+
+struct TuningEntry {
+  int32_t ndim;
+  int64_t max_mean_event;
+  int32_t group_queries;
+  int32_t xcd_chunk_log2;
+};
+
+inline constexpr TuningEntry kTuningTable[] = {
+    {2, 2048, 0, 6},
+    {2, 16384, 1, 2},
+    {2, 131072, 1, 5},
+    {2, INT64_MAX, 1, 3},
+    {3, 2048, 0, 4},
+    {3, 16384, 1, 5},
+    {3, 131072, 1, 6},
+    {3, INT64_MAX, 0, 5},
+    {0, INT64_MAX, 1, 4},  // other ndim
+};
+
+}  // namespace clue_rt

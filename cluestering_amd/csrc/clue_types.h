@@ -59,6 +59,7 @@ struct RunParams {
   int32_t exp_in_range; // every kernel exp argument within the radius lies in [-103, 0]
   int32_t wrapped[kMaxDim];
   float mparam[kMaxDim];
+  int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none
 };
 
 }  // namespace clue_dev

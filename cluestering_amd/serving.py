@@ -123,45 +123,65 @@ def make_pool(n_events: int, seed: int = 0xC1E0 + 5, lo: int = 256, hi: int = 26
 
 
 def run_stream(server: Server, pool: Sequence[ServedEvent], rate: float, n_events: int,
-               seed: int = 1) -> dict:
+               seed: int = 1, n_clients: int = 4) -> dict:
     """Submits n_events from the pool at Poisson arrival times of the given rate
-    (events/s) from one client thread while a collector thread waits for them.
-    Latency = completion time - scheduled arrival (same clock as the server)."""
+    (events/s) while a collector thread waits for them in arrival order. Arrival k
+    is submitted by client thread k mod n_clients, so one slow submit does not hold
+    back the next arrivals (ctypes releases the GIL during the submit's copy).
+    Latency = completion time - scheduled arrival (same clock as the server);
+    late_submits counts arrivals submitted more than 1 ms after their time."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=n_events)
     arrivals = np.cumsum(gaps) * 1e9
     order = rng.integers(0, len(pool), size=n_events)
     ids: List[int] = [0] * n_events
-    ready = threading.Semaphore(0)
+    ready = [threading.Event() for _ in range(n_events)]
     lat = np.zeros(n_events)
     done = np.zeros(n_events)
     batch_ev = np.zeros(n_events)
+    late = np.zeros(n_clients, np.int64)
+    errors: List[BaseException] = []
     t0 = now_ns() + 2_000_000
 
     def collector():
         for k in range(n_events):
-            ready.acquire()
+            ready[k].wait()
+            if errors:
+                return
             _, t = server.wait(ids[k])
             lat[k] = t["done_ns"] - (t0 + arrivals[k])
             done[k] = t["done_ns"]
             batch_ev[k] = t["batch_events"]
 
+    def client(c):
+        try:
+            for k in range(c, n_events, n_clients):
+                target = t0 + arrivals[k]
+                while True:
+                    dt = target - now_ns()
+                    if dt <= 0:
+                        break
+                    time.sleep(min(dt, 2e6) / 1e9 if dt > 200_000 else 0)
+                if now_ns() - target > 1_000_000:
+                    late[c] += 1
+                ev = pool[order[k]]
+                ids[k] = server.submit(ev.params, ev.data)
+                ready[k].set()
+        except BaseException as e:  # surface in the caller, release the collector
+            errors.append(e)
+            for r in ready:
+                r.set()
+
     th = threading.Thread(target=collector)
     th.start()
-    late = 0
-    for k in range(n_events):
-        target = t0 + arrivals[k]
-        while True:
-            dt = target - now_ns()
-            if dt <= 0:
-                break
-            time.sleep(min(dt, 2e6) / 1e9 if dt > 200_000 else 0)
-        if now_ns() - target > 1_000_000:
-            late += 1
-        ev = pool[order[k]]
-        ids[k] = server.submit(ev.params, ev.data)
-        ready.release()
+    clients = [threading.Thread(target=client, args=(c,)) for c in range(n_clients)]
+    for t in clients:
+        t.start()
+    for t in clients:
+        t.join()
     th.join()
+    if errors:
+        raise errors[0]
     span = (done.max() - t0) / 1e9
     pts = int(sum(pool[i].n for i in order))
     return {"rate": rate, "events": n_events, "points": pts,
@@ -170,4 +190,4 @@ def run_stream(server: Server, pool: Sequence[ServedEvent], rate: float, n_event
                            "p99": float(np.percentile(lat, 99)) / 1e6,
                            "mean": float(lat.mean()) / 1e6},
             "mean_batch_events": float(batch_ev.mean()),
-            "late_submits": late}
+            "late_submits": int(late.sum()), "clients": n_clients}

@@ -174,6 +174,21 @@ clue_status clue_set_profiling(clue_ctx* ctx, int32_t enabled);
 clue_status clue_stage_times(clue_ctx* ctx, const char** names, float* us, int32_t capacity,
                              int32_t* count);
 
+/* Launch tuning (no reference counterpart): choices that change the speed of the
+ * neighbour stages and never their results. By default each run takes the entry
+ * of the built-in table (measured on MI355X per ndim and mean event size by
+ * cluestering_amd/autotune.py) that matches its batch. */
+typedef struct {
+  int32_t group_queries;  /* 1: order each tile's queries by box class and position */
+  int32_t xcd_chunk_log2; /* deal 2^k consecutive work-item blocks per XCD turn; -1: none */
+  int32_t reserved[6];
+} clue_tuning;
+/* Fixes the tuning of the context's later runs and plans; NULL returns to the table. */
+clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning);
+/* The tuning a run of this batch would use (sizes: host, NULL = one event of n). */
+clue_status clue_get_tuning(clue_ctx* ctx, const clue_params* params, int32_t n,
+                            const uint32_t* sizes, int32_t n_events, clue_tuning* out);
+
 /* Graph capture of the batched pipeline: records every stage of one batch
  * (fixed pointers and sizes) in a hipGraph and replays it with one launch. */
 typedef struct clue_plan clue_plan;
@@ -243,6 +258,10 @@ clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a, const floa
 /* A plain device copy of `bytes` (multiple of 16), the calibration point of the
  * HBM traffic counters (FETCH_SIZE / WRITE_SIZE). No reference counterpart. */
 clue_status clue_debug_copy(clue_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Test hook: fills every workspace buffer of the context with 0xFF bytes (NaN
+ * floats, -1 integers), so that a later run cannot pass on values left by an
+ * earlier one. Waits. */
+clue_status clue_debug_poison(clue_ctx* ctx);
 
 #ifdef __cplusplus
 }

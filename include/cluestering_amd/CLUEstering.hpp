@@ -758,4 +758,38 @@ PointsHost<Ndim, T> read_csv(Queue& q, const std::string& path) {
   return h;
 }
 
+// read_output (utils/detail/read_csv.hpp:68-104): a clustering result file, one row
+// per point with Ndim coordinates, the weight and the cluster id after a header
+// line; the points come back marked clustered.
+template <std::size_t Ndim, class T = float>
+PointsHost<Ndim, T> read_output(Queue& q, const std::string& path) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("Could not open file: " + path);
+  std::string line;
+  std::getline(f, line);
+  std::vector<std::array<float, Ndim + 1>> rows;
+  std::vector<int32_t> ids;
+  while (std::getline(f, line)) {
+    if (line.empty()) continue;
+    std::stringstream ss(line);
+    std::array<float, Ndim + 1> r{};
+    std::string cell;
+    for (std::size_t k = 0; k < Ndim + 1; ++k) {
+      if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
+      r[k] = std::stof(cell);
+    }
+    if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
+    rows.push_back(r);
+    ids.push_back(static_cast<int32_t>(std::stoi(cell)));
+  }
+  PointsHost<Ndim, T> h(q, static_cast<int32_t>(rows.size()));
+  for (std::size_t i = 0; i < rows.size(); ++i) {
+    for (std::size_t d = 0; d < Ndim; ++d) h.coords(d)[i] = rows[i][d];
+    h.weights()[i] = rows[i][Ndim];
+    h.clusterIndexes()[i] = ids[i];
+  }
+  h.mark_clustered();
+  return h;
+}
+
 }  // namespace clue

@@ -179,6 +179,26 @@ static void test_device_points_and_clusters() {
   CHECK(dclusters.count(3) == clusters.count(3));
   auto free_fn = clue::get_clusters(q, d);
   CHECK(free_fn.to_host().indexes() == clusters.indexes());
+  // read_output on a result file written from these points.
+  const std::string out = "/tmp/cluestering_amd_read_output_test.csv";
+  {
+    std::FILE* fp = std::fopen(out.c_str(), "w");
+    std::fprintf(fp, "x0,x1,weight,cluster_ids\n");
+    for (int32_t i = 0; i < h.size(); ++i)
+      std::fprintf(fp, "%.9g,%.9g,%.9g,%d\n", h.coords(0)[i], h.coords(1)[i], h.weights()[i],
+                   h.clusterIndexes()[i]);
+    std::fclose(fp);
+  }
+  auto back2 = clue::read_output<2, float>(q, out);
+  CHECK(back2.clustered());
+  CHECK(back2.size() == h.size());
+  bool same_ids = true;
+  for (int32_t i = 0; i < h.size(); ++i)
+    same_ids = same_ids && back2.clusterIndexes()[i] == h.clusterIndexes()[i] &&
+               back2.coords(0)[i] == h.coords(0)[i];
+  CHECK(same_ids);
+  CHECK(back2.n_clusters() == h.n_clusters());
+  std::remove(out.c_str());
 }
 
 int main(int argc, char** argv) {

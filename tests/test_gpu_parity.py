@@ -90,6 +90,23 @@ def test_batched_gaussian(gpu_ctx, oracle_mod):
     assert gpu["n_clusters"] == ref.n_seeds
 
 
+@pytest.mark.parametrize("tuning", [(0, -1), (1, -1), (1, 0), (1, 4), (0, 6), (1, 12)])
+def test_tunings_do_not_change_results(gpu_ctx, oracle_mod, tuning):
+    """Every launch tuning (query grouping on/off, XCD chunking) gives the same bits:
+    a 2-D gaussian batch with dense tiles and a 3-D periodic exponential event."""
+    from cluestering_amd import _capi
+    group, xcd = tuning
+    gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd)
+    try:
+        for w, name in [(synthetic.batch2d_4096(n_events=12, lo=4096, hi=30000), "batch12"),
+                        (synthetic.detector3d_1m(n=1 << 15), "detector3d_32k")]:
+            # No values from an earlier run may survive in the workspace.
+            _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
+            _check(_workload_case(w, f"{name} tuning={tuning}"), gpu_ctx, oracle_mod)
+    finally:
+        gpu_ctx.set_tuning()
+
+
 @pytest.mark.parametrize("kernel", [K.FlatKernel(0.5), K.ExponentialKernel(0.5, 1.0),
                                     K.GaussianKernel(1.0, 1.0), K.FlatKernel(2.0),
                                     K.GaussianKernel(0.7, 1.3), K.ExponentialKernel(1.7, 0.8),
