@@ -79,7 +79,8 @@ class Tuning(C.Structure):
     _fields_ = [
         ("group_queries", C.c_int32),
         ("xcd_chunk_log2", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("block_waves", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

@@ -2,7 +2,7 @@
 
 For each ndim and mean-event-size bucket it builds a synthetic batch, replays it
 as a captured graph under every candidate tuning (query grouping on/off x XCD
-chunk sizes) and keeps the fastest. Tuning never changes results, and the sweep
+chunk sizes x waves per workgroup) and keeps the fastest. Tuning never changes results, and the sweep
 checks that too: every candidate's labels must equal the first candidate's.
 
     python -m cluestering_amd.autotune --out autotune.json [--write-table]
@@ -24,8 +24,9 @@ from . import CLUE_Convolutional_Kernels as K
 from . import synthetic
 from .device import Clusterer, Context, Plan, PointsDevice
 
-# Candidate tunings: (group_queries, xcd_chunk_log2).
-CANDIDATES: List[Tuple[int, int]] = [(g, x) for g in (0, 1) for x in (-1, 2, 3, 4, 5, 6)]
+# Candidate tunings: (group_queries, xcd_chunk_log2, block_waves).
+CANDIDATES: List[Tuple[int, int, int]] = [(g, x, b) for g in (0, 1) for x in (-1, 2, 4, 6, 8)
+                                          for b in (1, 2, 4)]
 
 # Buckets by mean event size: (upper bound of the bucket, event size range of the
 # synthetic batch). The last bucket is one large event.
@@ -75,8 +76,8 @@ def sweep_bucket(ctx: Context, ndim: int, bound: int, lo: int, hi: int, target_p
     coords, weight, sizes = _batch(ndim, lo, hi, target_points, rng)
     pts, algo, metric, kernel = _setup(ctx, ndim, coords, weight)
     results, ref_labels = [], None
-    for group, xcd in CANDIDATES:
-        ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd)
+    for group, xcd, waves in CANDIDATES:
+        ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd, block_waves=waves)
         plan = Plan(algo, pts, sizes, metric, kernel)
         plan.launch()
         ctx.synchronize()
@@ -84,7 +85,7 @@ def sweep_bucket(ctx: Context, ndim: int, bound: int, lo: int, hi: int, target_p
         if ref_labels is None:
             ref_labels = labels
         elif not np.array_equal(labels, ref_labels):
-            raise AssertionError(f"tuning {(group, xcd)} changed the labels")
+            raise AssertionError(f"tuning {(group, xcd, waves)} changed the labels")
         times = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -92,7 +93,7 @@ def sweep_bucket(ctx: Context, ndim: int, bound: int, lo: int, hi: int, target_p
             ctx.synchronize()
             times.append(time.perf_counter() - t0)
         plan.close()
-        results.append({"group_queries": group, "xcd_chunk_log2": xcd,
+        results.append({"group_queries": group, "xcd_chunk_log2": xcd, "block_waves": waves,
                         "ms": float(np.median(times) * 1e3)})
     ctx.set_tuning()
     best = min(results, key=lambda r: r["ms"])
@@ -105,14 +106,15 @@ def write_table(entries: List[Dict], path: str = TABLE) -> None:
     for e in sorted(entries, key=lambda e: (e["ndim"], e["max_mean_event"])):
         b = e["best"]
         bound = "INT64_MAX" if e["max_mean_event"] >= 2 ** 62 else str(e["max_mean_event"])
-        lines.append(f"    {{{e['ndim']}, {bound}, {b['group_queries']}, {b['xcd_chunk_log2']}}},")
-    lines.append("    {0, INT64_MAX, 1, 4},  // other ndim")
+        lines.append(f"    {{{e['ndim']}, {bound}, {b['group_queries']}, {b['xcd_chunk_log2']}, "
+                     f"{b['block_waves']}}},")
+    lines.append("    {0, INT64_MAX, 1, 4, 4},  // other ndim")
     text = (
         "// Built-in launch tuning, by ndim (0 = any) and mean event size (first match).\n"
         "// Regenerated from MI355X measurements by cluestering_amd/autotune.py.\n"
         "#pragma once\n\n#include <cstdint>\n\nnamespace clue_rt {\n\n"
         "struct TuningEntry {\n  int32_t ndim;\n  int64_t max_mean_event;\n"
-        "  int32_t group_queries;\n  int32_t xcd_chunk_log2;\n};\n\n"
+        "  int32_t group_queries;\n  int32_t xcd_chunk_log2;\n  int32_t block_waves;\n};\n\n"
         "inline constexpr TuningEntry kTuningTable[] = {\n" + "\n".join(lines) + "\n};\n\n"
         "}  // namespace clue_rt\n")
     with open(path, "w") as f:

@@ -340,11 +340,16 @@ clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning) {
   if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
   if (tuning && (tuning->xcd_chunk_log2 < -1 || tuning->xcd_chunk_log2 > 12))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "xcd_chunk_log2 must lie in [-1, 12]");
+  if (tuning && tuning->block_waves != 0 && tuning->block_waves != 1 &&
+      tuning->block_waves != 2 && tuning->block_waves != 4 && tuning->block_waves != 8 &&
+      tuning->block_waves != 16)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "block_waves must be 1, 2, 4, 8 or 16");
   ctx->tuned = tuning != nullptr;
   if (tuning) {
     ctx->tuning = clue_tuning{};
     ctx->tuning.group_queries = tuning->group_queries ? 1 : 0;
     ctx->tuning.xcd_chunk_log2 = tuning->xcd_chunk_log2;
+    ctx->tuning.block_waves = tuning->block_waves ? tuning->block_waves : 4;
   }
   return CLUE_OK;
 }

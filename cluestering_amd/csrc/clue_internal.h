@@ -64,6 +64,7 @@ struct BatchBuffers {
   int32_t* qperm;      // [n_total] density query order within tiles; NULL = sorted order
   int32_t* qperm_nh;   // [n_total] nearest-higher query order (may alias qperm)
   int32_t* qfine;      // [n_total] Morton order within tiles (input of both)
+  uint32_t* qkey;      // [n_total] packed query keys (Morton, class for dc, class for dm)
   float* srho;
   uint64_t* skey;      // (rho, tag) ordering key of every sorted point
   int32_t* snh;

@@ -30,11 +30,13 @@ clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events) {
       clue_tuning out{};
       out.group_queries = t.group_queries;
       out.xcd_chunk_log2 = t.xcd_chunk_log2;
+      out.block_waves = t.block_waves;
       return out;
     }
   clue_tuning out{};
   out.group_queries = 1;
   out.xcd_chunk_log2 = 4;
+  out.block_waves = 4;
   return out;
 }
 
@@ -141,6 +143,7 @@ RunParams make_run_params(const clue_params* p, const clue_points* pts, const Ba
   const int64_t n_total = bp.n_total;
   RunParams rp{};
   rp.xcd_chunk_log2 = bp.tuning.xcd_chunk_log2;
+  rp.block_waves = bp.tuning.block_waves > 0 ? bp.tuning.block_waves : 4;
   rp.D = p->ndim;
   rp.kernel = p->kernel;
   rp.metric = p->metric;
@@ -243,6 +246,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
       {S_QPERM, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc) ? sizeof(int32_t) * n : 0},
       {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
+      {S_QKEY, bp.tuning.group_queries ? sizeof(uint32_t) * n : 0},
       {S_SRHO, sizeof(float) * n},
       {S_SKEY, sizeof(uint64_t) * n},
       {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(float) * n : 0},
@@ -293,6 +297,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc) ? ws.buf[S_QPERM_NH].as<int32_t>()
                                                           : b.qperm;
   b.qfine = bp.tuning.group_queries ? ws.buf[S_QFINE].as<int32_t>() : nullptr;
+  b.qkey = bp.tuning.group_queries ? ws.buf[S_QKEY].as<uint32_t>() : nullptr;
   b.srho = ws.buf[S_SRHO].as<float>();
   b.skey = ws.buf[S_SKEY].as<uint64_t>();
   b.snh = ws.buf[S_SNH].as<int32_t>();

@@ -26,6 +26,10 @@ namespace clue_dev {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / kWave;
+// The neighbour stages take 1 to 16 waves per workgroup (a tuning choice): a
+// workgroup runs on one CU, so its consecutive work items (neighbouring tiles)
+// share that CU's scalar cache, while smaller workgroups spread more evenly.
+constexpr int kMaxNeighbourBlock = 1024;
 constexpr float kSqrt2 = 0x1.6a09e6p+0f;  // sqrtf(2.0f), correctly rounded
 
 enum { M_EUCLID = 0, M_WEUCLID = 1, M_PERIODIC = 2, M_MANHATTAN = 3, M_CHEBYSHEV = 4,
@@ -371,13 +375,10 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 // where the class says, per dimension (first three), whether the search box
 // reaches below and above the tile's bin; items then take 64 consecutive entries
 // of that order. Results do not depend on the order: a point's traversal is its
-// own, whichever lane runs it.
-// Two stable counting sorts per tile, one wave per tile, ranking by ballot:
-//   kOrderFine:  sorted order -> Morton order (shared by both stages);
-//   kOrderClass: Morton order -> (class, Morton) for the stage's radius.
+// own, whichever lane runs it. k_query_order builds both stages' orders, one
+// wave per tile (below).
 // ---------------------------------------------------------------------------
 constexpr int kQueryDims = 3;  // dimensions that enter the keys
-enum { kOrderFine = 0, kOrderClass = 1 };
 
 template <int D>
 __device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[D],
@@ -415,23 +416,89 @@ __device__ __forceinline__ uint32_t query_fine(const Geo& g, const float (&xi)[D
   return key;
 }
 
-template <int D, int Mode>
-__global__ __launch_bounds__(kBlock) void k_tile_order(
+// Makes the wave's global stores visible to the wave's later loads (lanes read
+// entries other lanes wrote): the stores complete before any later access.
+__device__ __forceinline__ void wave_memory_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Stable counting sort of the points at positions [b, e) of `order` (identity
+// when NULL) by a Bits-bit key, into out[b, e). One wave; cnt is its LDS
+// counter array of 2^Bits entries. Equal keys keep their input order: rounds and
+// lanes are visited in position order and ranked by ballot.
+template <int Bits, class KeyOf>
+__device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* order,
+                                                   int32_t* cnt, KeyOf key_of, int32_t* out) {
+  constexpr int kKeys = 1 << Bits;
+  const int lane = lane_id();
+  for (int c = lane; c < kKeys; c += kWave) cnt[c] = 0;
+  __builtin_amdgcn_wave_barrier();
+  for (int i0 = b; i0 < e; i0 += kWave) {
+    const int i = i0 + lane;
+    const bool valid = i < e;
+    const int s = valid ? (order ? order[i] : i) : 0;
+    const uint32_t k = valid ? key_of(s) : 0u;
+    const uint64_t peers = match_peers(k, Bits, valid);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lanemask_lt()) == 0) cnt[k] += __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+  }
+  int carry = b;
+  for (int c0 = 0; c0 < kKeys; c0 += kWave) {
+    const int c = c0 + lane;
+    const int v = c < kKeys ? cnt[c] : 0;
+    int x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    if (c < kKeys) cnt[c] = carry + x - v;
+    carry += __shfl(x, kWave - 1, 64);
+  }
+  __builtin_amdgcn_wave_barrier();
+  for (int i0 = b; i0 < e; i0 += kWave) {
+    const int i = i0 + lane;
+    const bool valid = i < e;
+    const int s = valid ? (order ? order[i] : i) : 0;
+    const uint32_t k = valid ? key_of(s) : 0u;
+    const uint64_t peers = match_peers(k, Bits, valid);
+    int pos = 0;
+    if (valid) pos = cnt[k] + __popcll(peers & lanemask_lt());
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lanemask_lt()) == 0) cnt[k] += __popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    if (valid) out[pos] = s;
+  }
+}
+
+// Both query orders of one tile per wave. Pass 1 packs each point's keys into
+// qkey (Morton index in bits 0-7, class for dc in 8-13, class for dm in 14-19);
+// then three stable counting sorts: sorted order -> Morton order (qfine), and
+// Morton order -> (class, Morton) for dc (qperm) and for dm (qperm_nh, skipped
+// when it aliases qperm).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_query_order(
     const RunParams rp, const EventDesc* __restrict__ events, int n_events,
     const float* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
-    const float* __restrict__ sx, int64_t cs, const float* __restrict__ ssig, float R,
-    const int32_t* __restrict__ in_order, int32_t* __restrict__ out_order) {
+    const float* __restrict__ sx, int64_t cs, const float* __restrict__ ssig,
+    uint32_t* __restrict__ qkey, int32_t* __restrict__ qfine, int32_t* __restrict__ qperm,
+    int32_t* __restrict__ qperm_nh) {
   constexpr int kD = D < kQueryDims ? D : kQueryDims;
-  constexpr int kKeyBits = Mode == kOrderFine ? (8 / kD) * kD : 2 * kD;
-  constexpr int kKeys = 1 << kKeyBits;
-  __shared__ int32_t base[kWaves][kKeys];
+  constexpr int kFineBits = (8 / kD) * kD;
+  constexpr int kClassBits = 2 * kD;
+  __shared__ int32_t cnt[kWaves][1 << kFineBits];
   const int w = threadIdx.x / kWave;
   const int t = blockIdx.x * kWaves + w;
   if (t >= n_tiles) return;
   const int lane = lane_id();
   const int b = toff[t], e = toff[t + 1];
-  if (e - b <= kWave) {  // one item: order is irrelevant
-    if (b + lane < e) out_order[b + lane] = b + lane;
+  if (e - b <= kWave) {  // one item: the order is irrelevant
+    if (b + lane < e) {
+      qperm[b + lane] = b + lane;
+      qperm_nh[b + lane] = b + lane;
+    }
     return;
   }
   // Event of tile t: last event with t_base <= t.
@@ -454,62 +521,27 @@ __global__ __launch_bounds__(kBlock) void k_tile_order(
       r /= ev.nper;
     }
   }
-  // Point at position i of the input order, and its key.
-  auto point = [&](int i) { return in_order ? in_order[i] : i; };
-  auto key = [&](int s, bool valid) -> uint32_t {
-    if (!valid) return 0u;
+  for (int s = b + lane; s < e; s += kWave) {
     float xi[D], si[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       xi[d] = sx[d * cs + s];
       si[d] = rp.has_sigma ? ssig[d * cs + s] : 0.0f;
     }
-    if constexpr (Mode == kOrderFine)
-      return query_fine<D>(g, xi, tb);
-    else
-      return query_class<D>(g, xi, si, R, tb);
-  };
-  // Pass 1: key counts.
-  for (int c = lane; c < kKeys; c += kWave) base[w][c] = 0;
-  __builtin_amdgcn_wave_barrier();
-  for (int i0 = b; i0 < e; i0 += kWave) {
-    const int i = i0 + lane;
-    const bool valid = i < e;
-    const uint32_t c = key(valid ? point(i) : 0, valid);
-    const uint64_t peers = match_peers(c, kKeyBits, valid);
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
-    __builtin_amdgcn_wave_barrier();
+    const uint32_t fine = query_fine<D>(g, xi, tb);
+    const uint32_t c_dc = query_class<D>(g, xi, si, rp.dc, tb);
+    const uint32_t c_dm = (qperm_nh != qperm) ? query_class<D>(g, xi, si, rp.dm, tb) : c_dc;
+    qkey[s] = fine | (c_dc << 8) | (c_dm << 14);
   }
-  // Exclusive scan of the counts, 64 keys per round.
-  int carry = b;
-  for (int c0 = 0; c0 < kKeys; c0 += kWave) {
-    const int c = c0 + lane;
-    const int v = c < kKeys ? base[w][c] : 0;
-    int x = v;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const int y = __shfl_up(x, off, 64);
-      if (lane >= off) x += y;
-    }
-    if (c < kKeys) base[w][c] = carry + x - v;
-    carry += __shfl(x, kWave - 1, 64);
-  }
-  __builtin_amdgcn_wave_barrier();
-  // Pass 2: scatter in key order; equal keys keep the input order.
-  for (int i0 = b; i0 < e; i0 += kWave) {
-    const int i = i0 + lane;
-    const bool valid = i < e;
-    const int s = valid ? point(i) : 0;
-    const uint32_t c = key(s, valid);
-    const uint64_t peers = match_peers(c, kKeyBits, valid);
-    int pos = 0;
-    if (valid) pos = base[w][c] + __popcll(peers & lanemask_lt());
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lanemask_lt()) == 0) base[w][c] += __popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    if (valid) out_order[pos] = s;
-  }
+  wave_memory_sync();  // qkey of every point of the tile
+  int32_t* c = cnt[w];
+  wave_counting_sort<kFineBits>(b, e, nullptr, c, [&](int s) { return qkey[s] & 0xFFu; }, qfine);
+  wave_memory_sync();  // qfine, the input order of the class sorts
+  wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 8) & 63u; },
+                                 qperm);
+  if (qperm_nh != qperm)
+    wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 14) & 63u; },
+                                   qperm_nh);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +587,7 @@ struct Unroll {
 // the additions keep the candidate order.
 // ---------------------------------------------------------------------------
 template <int D, int M, int K>
-__global__ __launch_bounds__(kBlock) void k_density(
+__global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
     const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
@@ -563,7 +595,8 @@ __global__ __launch_bounds__(kBlock) void k_density(
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
     float* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
-  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
+                   threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -663,7 +696,7 @@ __device__ __forceinline__ float prune_threshold(float eff_thr, float delta) {
 }
 
 template <int D, int M>
-__global__ __launch_bounds__(kBlock) void k_nearest_higher(
+__global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
     const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
@@ -672,7 +705,8 @@ __global__ __launch_bounds__(kBlock) void k_nearest_higher(
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ qperm,
     int32_t* __restrict__ snh, float* __restrict__ sdelta) {
   constexpr int U = Unroll<D>::U;
-  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * kWaves + threadIdx.x / kWave;
+  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
+                   threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
@@ -835,13 +869,14 @@ struct NeighbourLaunch {
   const BatchBuffers* bb;
   int64_t cs;
   unsigned grid;
+  unsigned block;  // threads per workgroup of the neighbour stages
   hipStream_t s;
 };
 
 template <int D, int M, int K>
 static void launch_density_dmk(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
-  hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
+  hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
                      b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
                      b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
 }
@@ -863,7 +898,7 @@ static void launch_density_dm(const NeighbourLaunch& L) {
 template <int D, int M>
 static void launch_nh_dm(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
-  hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(kBlock), 0, L.s, *L.rp,
+  hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp,
                      b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
                      b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm_nh,
                      b.snh, b.sdelta);
@@ -895,32 +930,27 @@ static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
 }
 
 template <int D>
-static void launch_query_order_d(const NeighbourLaunch& L, int32_t* out, float R) {
+static void launch_query_order_d(const NeighbourLaunch& L) {
   const BatchBuffers& b = *L.bb;
   const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
-  if (out == b.qfine)
-    hipLaunchKernelGGL((k_tile_order<D, kOrderFine>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp,
-                       b.events, b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
-                       b.sorted.sigma[0], R, nullptr, b.qfine);
-  else
-    hipLaunchKernelGGL((k_tile_order<D, kOrderClass>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp,
-                       b.events, b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
-                       b.sorted.sigma[0], R, b.qfine, out);
+  hipLaunchKernelGGL((k_query_order<D>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
+                     b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
+                     b.sorted.sigma[0], b.qkey, b.qfine, b.qperm, b.qperm_nh);
 }
 
-// out == qfine: the Morton order; else the (class, Morton) order for radius R.
-static void launch_query_order(const NeighbourLaunch& L, int32_t* out, float R) {
+// The query orders of both neighbour stages (qperm for dc, qperm_nh for dm).
+static void launch_query_order(const NeighbourLaunch& L) {
   switch (L.rp->D) {
-    case 1: launch_query_order_d<1>(L, out, R); break;
-    case 2: launch_query_order_d<2>(L, out, R); break;
-    case 3: launch_query_order_d<3>(L, out, R); break;
-    case 4: launch_query_order_d<4>(L, out, R); break;
-    case 5: launch_query_order_d<5>(L, out, R); break;
-    case 6: launch_query_order_d<6>(L, out, R); break;
-    case 7: launch_query_order_d<7>(L, out, R); break;
-    case 8: launch_query_order_d<8>(L, out, R); break;
-    case 9: launch_query_order_d<9>(L, out, R); break;
-    default: launch_query_order_d<10>(L, out, R); break;
+    case 1: launch_query_order_d<1>(L); break;
+    case 2: launch_query_order_d<2>(L); break;
+    case 3: launch_query_order_d<3>(L); break;
+    case 4: launch_query_order_d<4>(L); break;
+    case 5: launch_query_order_d<5>(L); break;
+    case 6: launch_query_order_d<6>(L); break;
+    case 7: launch_query_order_d<7>(L); break;
+    case 8: launch_query_order_d<8>(L); break;
+    case 9: launch_query_order_d<9>(L); break;
+    default: launch_query_order_d<10>(L); break;
   }
 }
 
@@ -987,14 +1017,12 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   L.rp = &rp;
   L.bb = &bb;
   L.cs = bb.sorted_stride;
-  const int64_t blocks = (bb.max_items + kWaves - 1) / kWaves;
+  const int waves = rp.block_waves;
+  L.block = static_cast<unsigned>(waves * kWave);
+  const int64_t blocks = (bb.max_items + waves - 1) / waves;
   L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
-  if (bb.qperm) {  // query orders of the two neighbour stages
-    launch_query_order(L, bb.qfine, 0.0f);
-    launch_query_order(L, bb.qperm, rp.dc);
-    if (bb.qperm_nh != bb.qperm) launch_query_order(L, bb.qperm_nh, rp.dm);
-  }
+  if (bb.qperm) launch_query_order(L);  // query orders of the two neighbour stages
 
   // (b), (c)
   mark(kStageDensity);

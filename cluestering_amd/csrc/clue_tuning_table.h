@@ -11,18 +11,19 @@ struct TuningEntry {
   int64_t max_mean_event;
   int32_t group_queries;
   int32_t xcd_chunk_log2;
+  int32_t block_waves;
 };
 
 inline constexpr TuningEntry kTuningTable[] = {
-    {2, 2048, 0, 6},
-    {2, 16384, 1, 2},
-    {2, 131072, 1, 5},
-    {2, INT64_MAX, 1, 3},
-    {3, 2048, 0, 4},
-    {3, 16384, 1, 5},
-    {3, 131072, 1, 6},
-    {3, INT64_MAX, 0, 5},
-    {0, INT64_MAX, 1, 4},  // other ndim
+    {2, 2048, 0, 6, 1},
+    {2, 16384, 1, 4, 1},
+    {2, 131072, 1, 6, 1},
+    {2, INT64_MAX, 0, 6, 1},
+    {3, 2048, 0, 8, 1},
+    {3, 16384, 1, 8, 1},
+    {3, 131072, 1, 8, 1},
+    {3, INT64_MAX, 0, 6, 2},
+    {0, INT64_MAX, 1, 4, 4},  // other ndim
 };
 
 }  // namespace clue_rt

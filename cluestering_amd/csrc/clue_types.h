@@ -60,6 +60,7 @@ struct RunParams {
   int32_t wrapped[kMaxDim];
   float mparam[kMaxDim];
   int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none
+  int32_t block_waves;     // neighbour stages: waves per workgroup (1, 2, 4, 8 or 16)
 };
 
 }  // namespace clue_dev

@@ -66,15 +66,17 @@ class Context:
         return d
 
     def set_tuning(self, group_queries: Optional[bool] = None,
-                   xcd_chunk_log2: Optional[int] = None) -> None:
+                   xcd_chunk_log2: Optional[int] = None,
+                   block_waves: Optional[int] = None) -> None:
         """Fix the launch tuning of later runs (speed only; results never change).
         With no arguments the built-in table chooses again."""
-        if group_queries is None and xcd_chunk_log2 is None:
+        if group_queries is None and xcd_chunk_log2 is None and block_waves is None:
             _capi.check(self._lib.clue_set_tuning(self.handle, None))
             return
         t = _capi.Tuning()
         t.group_queries = 1 if group_queries or group_queries is None else 0
         t.xcd_chunk_log2 = 4 if xcd_chunk_log2 is None else int(xcd_chunk_log2)
+        t.block_waves = 4 if block_waves is None else int(block_waves)
         _capi.check(self._lib.clue_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self, params: _capi.Params, sizes: Sequence[int]) -> dict:
@@ -84,7 +86,8 @@ class Context:
         _capi.check(self._lib.clue_get_tuning(self.handle, C.byref(params), int(sz.sum()),
                                               sz.ctypes.data_as(C.POINTER(C.c_uint32)),
                                               len(sz), C.byref(t)))
-        return {"group_queries": bool(t.group_queries), "xcd_chunk_log2": int(t.xcd_chunk_log2)}
+        return {"group_queries": bool(t.group_queries), "xcd_chunk_log2": int(t.xcd_chunk_log2),
+                "block_waves": int(t.block_waves)}
 
     def set_profiling(self, enabled: bool) -> None:
         _capi.check(self._lib.clue_set_profiling(self.handle, 1 if enabled else 0))

@@ -90,13 +90,15 @@ def test_batched_gaussian(gpu_ctx, oracle_mod):
     assert gpu["n_clusters"] == ref.n_seeds
 
 
-@pytest.mark.parametrize("tuning", [(0, -1), (1, -1), (1, 0), (1, 4), (0, 6), (1, 12)])
+@pytest.mark.parametrize("tuning", [(0, -1, 4), (1, -1, 4), (1, 0, 4), (1, 4, 4), (0, 6, 4),
+                                    (1, 12, 4), (1, 2, 8), (1, 4, 16), (0, -1, 16)])
 def test_tunings_do_not_change_results(gpu_ctx, oracle_mod, tuning):
-    """Every launch tuning (query grouping on/off, XCD chunking) gives the same bits:
-    a 2-D gaussian batch with dense tiles and a 3-D periodic exponential event."""
+    """Every launch tuning (query grouping on/off, XCD chunking, waves per
+    workgroup) gives the same bits: a 2-D gaussian batch with dense tiles and a
+    3-D periodic exponential event."""
     from cluestering_amd import _capi
-    group, xcd = tuning
-    gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd)
+    group, xcd, waves = tuning
+    gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd, block_waves=waves)
     try:
         for w, name in [(synthetic.batch2d_4096(n_events=12, lo=4096, hi=30000), "batch12"),
                         (synthetic.detector3d_1m(n=1 << 15), "detector3d_32k")]:
