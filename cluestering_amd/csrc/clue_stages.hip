@@ -575,10 +575,12 @@ __device__ __forceinline__ float kernel_value(float v, const RunParams& rp) {
 }
 
 // Candidates are walked in blocks of U so that their wave-uniform data arrives
-// in a few wide scalar loads.
+// in a few wide scalar loads (measured on MI355X: 8 for density, 16 for
+// nearest-higher, whose blocks carry less work per candidate).
 template <int D>
 struct Unroll {
   static constexpr int U = D <= 3 ? 8 : 4;
+  static constexpr int U_NH = D <= 3 ? 16 : 4;
 };
 
 // ---------------------------------------------------------------------------
@@ -704,7 +706,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
     const uint32_t* __restrict__ stag, const float* __restrict__ srho,
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ qperm,
     int32_t* __restrict__ snh, float* __restrict__ sdelta) {
-  constexpr int U = Unroll<D>::U;
+  constexpr int U = Unroll<D>::U_NH;
   const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
                    threadIdx.x / kWave;
   if (item >= *n_items) return;
@@ -765,17 +767,17 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
     if (!in) return;  // lanes whose box does not hold this tile idle through it
     int j = j0;
     for (; j + U <= j1; j += U) {
+      // m: the smallest distance among the block's higher candidates (vector
+      // selects and minima instead of per-candidate scalar mask logic).
       float v[U];
-      bool pass[U];
-      bool any = false;
+      float m = __builtin_inff();
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
-        pass[k] = (skey[j + k] > key_i) & (v[k] <= bound);
-        any = any | pass[k];
+        m = __builtin_fminf(m, (skey[j + k] > key_i) ? v[k] : __builtin_inff());
       }
       // Most blocks hold no candidate for any lane: one wave-wide test skips them.
-      if (ballot(any) != 0ull) {
+      if (ballot(m <= bound) != 0ull) {
         float rj[U];
         uint32_t tj[U];
 #pragma unroll
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
         }
 #pragma unroll
         for (int k = 0; k < U; ++k)
-          if (pass[k] && v[k] <= bound) consider(v[k], rj[k], tj[k], j + k);
+          if ((skey[j + k] > key_i) && v[k] <= bound) consider(v[k], rj[k], tj[k], j + k);
       }
     }
     for (; j < j1; ++j) {
