@@ -75,6 +75,30 @@ class Results(C.Structure):
     ]
 
 
+class ParamsF64(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("kernel", C.c_int32),
+        ("metric", C.c_int32),
+        ("points_per_tile", C.c_int32),
+        ("wrapped", C.c_int32 * MAX_DIM),
+        ("dc", C.c_double),
+        ("rhoc", C.c_double),
+        ("dm", C.c_double),
+        ("seed_dc", C.c_double),
+        ("kparam", C.c_double * 2),
+        ("mparam", C.c_double * MAX_DIM),
+    ]
+
+
+class PointsF64(C.Structure):
+    _fields_ = Points._fields_
+
+
+class ResultsF64(C.Structure):
+    _fields_ = Results._fields_
+
+
 class Tuning(C.Structure):
     _fields_ = [
         ("group_queries", C.c_int32),
@@ -137,6 +161,16 @@ _SIGNATURES = {
     "clue_make_clusters_host": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p,
                                             C.c_void_p, C.POINTER(C.c_uint32), C.c_int32,
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
+    "clue_validate_params_f64": (C.c_int32, [C.POINTER(ParamsF64)]),
+    "clue_make_clusters_f64": (C.c_int32, [C.c_void_p, C.POINTER(ParamsF64), C.POINTER(PointsF64),
+                                           C.POINTER(ResultsF64)]),
+    "clue_make_clusters_batched_f64": (C.c_int32, [C.c_void_p, C.POINTER(ParamsF64),
+                                                   C.POINTER(PointsF64), C.POINTER(C.c_uint32),
+                                                   C.c_int32, C.POINTER(ResultsF64), C.c_void_p,
+                                                   C.POINTER(C.c_int64)]),
+    "clue_make_clusters_host_f64": (C.c_int32, [C.c_void_p, C.POINTER(ParamsF64), C.c_int32,
+                                                C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
+                                                C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "clue_get_seeds": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "clue_get_n_clusters": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "clue_make_association": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
@@ -162,6 +196,8 @@ _SIGNATURES = {
                                     C.c_void_p, C.c_void_p]),
     "clue_debug_copy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "clue_debug_poison": (C.c_int32, [C.c_void_p]),
+    "clue_debug_math_f64": (C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "clue_server_create": (C.c_int32, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "clue_server_destroy": (C.c_int32, [C.c_void_p]),
     "clue_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.c_int32, C.c_void_p,
@@ -218,8 +254,9 @@ def make_params(ndim: int, dc: float, rhoc: float, dm: Optional[float] = None,
                 seed_dc: Optional[float] = None, kernel: int = KERNEL_FLAT,
                 kparams: Sequence[float] = (0.5,), metric: int = METRIC_EUCLIDEAN,
                 mparams: Sequence[float] = (), wrapped: Sequence[int] = (),
-                points_per_tile: int = 0) -> Params:
-    p = Params()
+                points_per_tile: int = 0, dtype=np.float32):
+    """clue_params, or clue_params_f64 for dtype float64 (Clusterer<Ndim, double>)."""
+    p = ParamsF64() if np.dtype(dtype) == np.float64 else Params()
     p.ndim = int(ndim)
     p.kernel = int(kernel)
     p.metric = int(metric)

@@ -71,9 +71,10 @@ clue_status read_stage_times(hipEvent_t* ev, const bool* recorded, const char** 
   return CLUE_OK;
 }
 
-clue_status run_batch(clue_ctx* ctx, const clue_params* p, const clue_points* pts,
-                      const uint32_t* sizes, int32_t n_events, clue_results* res,
-                      int32_t* event_of_cluster) {
+template <class P, class Pts, class Res>
+clue_status run_batch(clue_ctx* ctx, const P* p, const Pts* pts, const uint32_t* sizes,
+                      int32_t n_events, Res* res, int32_t* event_of_cluster) {
+  using T = value_t<P>;
   clue_status st = validate(p);
   if (st) return st;
   if (!res || !res->cluster_index) return fail(CLUE_ERR_INVALID_ARGUMENT, "cluster_index is required");
@@ -88,12 +89,12 @@ clue_status run_batch(clue_ctx* ctx, const clue_params* p, const clue_points* pt
   if (st) return st;
   st = check_points(p, pts, bp.n_total);
   if (st) return st;
-  BatchBuffers bb;
-  ResultArrays ra;
+  BatchBuffersT<T> bb;
+  ResultArraysT<T> ra;
   st = prepare_batch(ctx->ws, bp, p, pts, res, ctx->stream, &bb, &ra);
   if (st) return st;
   ra.event_of_cluster = event_of_cluster;
-  const RunParams rp = make_run_params(p, pts, bp);
+  const RunParamsT<T> rp = make_run_params(p, pts, bp);
   StageHook hook{mark_stage, ctx};
   if (ctx->profiling && !ctx->ev_ready) {
     for (auto& e : ctx->ev) CLUE_HIP(hipEventCreate(&e));
@@ -214,11 +215,16 @@ clue_status clue_make_clusters_batched(clue_ctx* ctx, const clue_params* params,
   return CLUE_OK;
 }
 
-clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, int32_t n,
-                                    const float* host_data, int32_t* host_cluster_index,
-                                    const uint32_t* sizes, int32_t n_events,
-                                    int32_t* host_is_seed, float* host_rho,
-                                    int32_t* host_nearest_higher) {
+}  // extern "C"
+
+namespace {
+
+// Host in, host out, in the precision of P.
+template <class P, class Pts, class Res>
+clue_status run_host(clue_ctx* ctx, const P* params, int32_t n, const value_t<P>* host_data,
+                     int32_t* host_cluster_index, const uint32_t* sizes, int32_t n_events,
+                     int32_t* host_is_seed, value_t<P>* host_rho, int32_t* host_nearest_higher) {
+  using T = value_t<P>;
   if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
   clue_status st = validate(params);
   if (st) return st;
@@ -227,16 +233,16 @@ clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, in
   const int D = params->ndim;
   ScopedDevice dev(ctx->device);
   Workspace& ws = ctx->ws;
-  const size_t col = sizeof(float) * static_cast<size_t>(n);
+  const size_t col = sizeof(T) * static_cast<size_t>(n);
   CLUE_HIP(ws.buf[S_HIN].reserve(col * (D + 1)));
   CLUE_HIP(ws.buf[S_HCL].reserve(sizeof(int32_t) * static_cast<size_t>(n)));
-  float* din = ws.buf[S_HIN].as<float>();
+  T* din = ws.buf[S_HIN].as<T>();
   CLUE_HIP(hipMemcpyAsync(din, host_data, col * (D + 1), hipMemcpyHostToDevice, ctx->stream));
-  clue_points pts{};
+  Pts pts{};
   pts.n = n;
   for (int d = 0; d < D; ++d) pts.coords[d] = din + static_cast<size_t>(n) * d;
   pts.weight = din + static_cast<size_t>(n) * D;
-  clue_results res{};
+  Res res{};
   res.cluster_index = ws.buf[S_HCL].as<int32_t>();
   const uint32_t one = static_cast<uint32_t>(n);
   if (!sizes) {
@@ -251,12 +257,61 @@ clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, in
     CLUE_HIP(hipMemcpyAsync(host_is_seed, ws.buf[S_ISSEED].p, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, ctx->stream));
   if (host_rho)
-    CLUE_HIP(hipMemcpyAsync(host_rho, ws.buf[S_RHO].p, sizeof(float) * n, hipMemcpyDeviceToHost,
+    CLUE_HIP(hipMemcpyAsync(host_rho, ws.buf[S_RHO].p, sizeof(T) * n, hipMemcpyDeviceToHost,
                             ctx->stream));
   if (host_nearest_higher)
     CLUE_HIP(hipMemcpyAsync(host_nearest_higher, ws.buf[S_NH].p, sizeof(int32_t) * n,
                             hipMemcpyDeviceToHost, ctx->stream));
   CLUE_HIP(hipStreamSynchronize(ctx->stream));
+  return CLUE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, int32_t n,
+                                    const float* host_data, int32_t* host_cluster_index,
+                                    const uint32_t* sizes, int32_t n_events,
+                                    int32_t* host_is_seed, float* host_rho,
+                                    int32_t* host_nearest_higher) {
+  return run_host<clue_params, clue_points, clue_results>(ctx, params, n, host_data,
+                                                          host_cluster_index, sizes, n_events,
+                                                          host_is_seed, host_rho,
+                                                          host_nearest_higher);
+}
+
+clue_status clue_make_clusters_host_f64(clue_ctx* ctx, const clue_params_f64* params, int32_t n,
+                                        const double* host_data, int32_t* host_cluster_index,
+                                        const uint32_t* sizes, int32_t n_events,
+                                        int32_t* host_is_seed, double* host_rho,
+                                        int32_t* host_nearest_higher) {
+  return run_host<clue_params_f64, clue_points_f64, clue_results_f64>(
+      ctx, params, n, host_data, host_cluster_index, sizes, n_events, host_is_seed, host_rho,
+      host_nearest_higher);
+}
+
+clue_status clue_validate_params_f64(const clue_params_f64* params) { return validate(params); }
+
+clue_status clue_make_clusters_f64(clue_ctx* ctx, const clue_params_f64* params,
+                                   const clue_points_f64* points, clue_results_f64* results) {
+  if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
+  if (!points) return fail(CLUE_ERR_INVALID_ARGUMENT, "null points");
+  if (points->n < 1) return fail(CLUE_ERR_INVALID_ARGUMENT, "the number of points must be positive");
+  const uint32_t n = static_cast<uint32_t>(points->n);
+  clue_status st = run_batch(ctx, params, points, &n, 1, results, nullptr);
+  if (st) return st;
+  return clue_synchronize(ctx);
+}
+
+clue_status clue_make_clusters_batched_f64(clue_ctx* ctx, const clue_params_f64* params,
+                                           const clue_points_f64* points, const uint32_t* sizes,
+                                           int32_t n_events, clue_results_f64* results,
+                                           int32_t* event_of_cluster, int64_t* n_clusters) {
+  if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
+  clue_status st = run_batch(ctx, params, points, sizes, n_events, results, event_of_cluster);
+  if (st) return st;
+  if (n_clusters) return clue_get_n_clusters(ctx, n_clusters);
   return CLUE_OK;
 }
 
@@ -509,6 +564,20 @@ extern "C" clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a,
   if (!ctx || !a || (out_div && !b)) return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
   ScopedDevice dev(ctx->device);
   clue_status st = clue_debug_math_impl(ctx->stream, n, a, b, out_sqrt, out_exp, out_div);
+  if (st) return fail(st, "debug math kernel failed");
+  return CLUE_OK;
+}
+
+extern "C" clue_status clue_debug_math_f64_impl(void* stream, int32_t n, const double* a,
+                                                const double* b, double* out_sqrt,
+                                                double* out_exp, double* out_div);
+
+extern "C" clue_status clue_debug_math_f64(clue_ctx* ctx, int32_t n, const double* a,
+                                           const double* b, double* out_sqrt, double* out_exp,
+                                           double* out_div) {
+  if (!ctx || !a || (out_div && !b)) return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
+  ScopedDevice dev(ctx->device);
+  clue_status st = clue_debug_math_f64_impl(ctx->stream, n, a, b, out_sqrt, out_exp, out_div);
   if (st) return fail(st, "debug math kernel failed");
   return CLUE_OK;
 }

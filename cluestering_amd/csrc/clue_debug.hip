@@ -20,6 +20,17 @@ __global__ void k_debug_math(int n, const float* __restrict__ a, const float* __
   if (out_div) out_div[i] = x / b[i];
 }
 
+__global__ void k_debug_math_f64(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                 double* __restrict__ out_sqrt, double* __restrict__ out_exp,
+                                 double* __restrict__ out_div) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i];
+  if (out_sqrt) out_sqrt[i] = clue_dev::sqrt_rn(x);
+  if (out_exp) out_exp[i] = clue_dev::expd_cw(x);
+  if (out_div) out_div[i] = x / b[i];
+}
+
 // A plain streaming copy, the reference point for the HBM traffic counters.
 __global__ void k_debug_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -46,6 +57,17 @@ extern "C" clue_status clue_debug_math_impl(void* stream, int32_t n, const float
   if (n <= 0) return CLUE_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_debug_math, dim3((n + 255) / 256), dim3(256), 0, s, n, a, b, out_sqrt,
+                     out_exp, out_div);
+  if (hipGetLastError() != hipSuccess) return CLUE_ERR_HIP;
+  return hipStreamSynchronize(s) == hipSuccess ? CLUE_OK : CLUE_ERR_HIP;
+}
+
+extern "C" clue_status clue_debug_math_f64_impl(void* stream, int32_t n, const double* a,
+                                                const double* b, double* out_sqrt,
+                                                double* out_exp, double* out_div) {
+  if (n <= 0) return CLUE_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_debug_math_f64, dim3((n + 255) / 256), dim3(256), 0, s, n, a, b, out_sqrt,
                      out_exp, out_div);
   if (hipGetLastError() != hipSuccess) return CLUE_ERR_HIP;
   return hipStreamSynchronize(s) == hipSuccess ? CLUE_OK : CLUE_ERR_HIP;

@@ -37,6 +37,22 @@ __device__ __forceinline__ int wave_max_i(int v) {
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
   return v;
 }
+__device__ __forceinline__ long long wave_min_i(long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ long long wave_max_i(long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Peers: the set of active lanes whose key equals this lane's key, built from one
@@ -87,6 +103,42 @@ __device__ __forceinline__ float expf_cw(float x) {
   return (xin != xin) ? xin : e;
 }
 
+// Double precision: the same method with a 29-bit ln2_hi (k*ln2_hi exact for
+// |k| <= 1075) and the degree-13 Taylor polynomial; the same constants and
+// operation order as the CPU statement. The core is for x in [-745.13, 709.78].
+__device__ __forceinline__ double expd_cw_core(double x) {
+  const double log2e = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42ff000000p-1;
+  const double ln2_lo = -0x1.718432a1b0e26p-35;
+  const double k = __builtin_rint(x * log2e);
+  double r = __builtin_fma(-k, ln2_hi, x);
+  r = __builtin_fma(-k, ln2_lo, r);
+  double p = 0x1.6124613a86d09p-33;
+  p = __builtin_fma(p, r, 0x1.1eed8eff8d898p-29);
+  p = __builtin_fma(p, r, 0x1.ae64567f544e4p-26);
+  p = __builtin_fma(p, r, 0x1.27e4fb7789f5cp-22);
+  p = __builtin_fma(p, r, 0x1.71de3a556c734p-19);
+  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-16);
+  p = __builtin_fma(p, r, 0x1.a01a01a01a01ap-13);
+  p = __builtin_fma(p, r, 0x1.6c16c16c16c17p-10);
+  p = __builtin_fma(p, r, 0x1.1111111111111p-7);
+  p = __builtin_fma(p, r, 0x1.5555555555555p-5);
+  p = __builtin_fma(p, r, 0x1.5555555555555p-3);
+  p = __builtin_fma(p, r, 0x1.0000000000000p-1);
+  p = __builtin_fma(p, r, 1.0);
+  p = __builtin_fma(p, r, 1.0);
+  return __builtin_ldexp(p, static_cast<int>(k));
+}
+
+__device__ __forceinline__ double expd_cw(double x) {
+  const double lo = -0x1.74910d52d3052p+9, hi = 0x1.62e42fefa39efp+9;
+  const double xin = x;
+  double e = expd_cw_core(__builtin_fmin(__builtin_fmax(x, lo), hi));
+  e = (xin > hi) ? __builtin_inf() : e;
+  e = (xin < lo) ? 0.0 : e;
+  return (xin != xin) ? xin : e;
+}
+
 // Correctly rounded square root. For s >= 2^-96 the hardware estimate r (within
 // 1 ulp) is settled by Tuckerman's rounding test on its neighbours r - ulp and
 // r + ulp, whose fma residuals s - r' * r carry the sign of the exact error
@@ -100,6 +152,20 @@ __device__ __forceinline__ float sqrt_rn(float s) {
   q = (__builtin_fmaf(-rp, r, s) > 0.0f) ? rp : q;
   if (__builtin_expect(!(s >= 0x1p-96f), 0)) q = __builtin_sqrtf(s);
   return q;
+}
+
+// Double: the compiler's IEEE sequence (the hardware estimate is not correctly
+// rounded in double).
+__device__ __forceinline__ double sqrt_rn(double s) { return __builtin_sqrt(s); }
+
+// Ordering key of a double density, for the nearest-higher test in double: the
+// high bits order rho (-0 folded onto +0); NaN is the minimum as a candidate and
+// the maximum as a query. The tag comparison is done separately (it does not fit).
+__device__ __forceinline__ uint64_t rho_key_f64(double r, bool query) {
+  if (r != r) return query ? ~0ull : 0ull;
+  if (r == 0.0) r = 0.0;
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(r));
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
 }
 
 // Ordering key of (rho, tag) such that, for a query point i and a candidate j,
@@ -117,26 +183,34 @@ __device__ __forceinline__ uint64_t rho_tag_key(float r, uint32_t tag, bool quer
   return (static_cast<uint64_t>(o) << 32) | (r > 0.0f ? tag : 0u);
 }
 
-// static_cast<int>(float) with the x86 semantics the reference's serial backend
-// runs with (cvttss2si): NaN and out-of-range values give INT_MIN.
+// static_cast<int>(float or double) with the x86 semantics the reference's
+// serial backend runs with (cvttss2si / cvttsd2si): NaN and out-of-range values
+// give INT_MIN.
 __device__ __forceinline__ int cvt_trunc(float q) {
   if (!(q >= -2147483648.0f) || !(q < 2147483648.0f)) return INT32_MIN;
   return static_cast<int>(q);
 }
+__device__ __forceinline__ int cvt_trunc(double q) {
+  if (!(q >= -2147483648.0) || !(q < 2147483648.0)) return INT32_MIN;
+  return static_cast<int>(q);
+}
 
 // Tile geometry of one event (TilesView.hpp:38-104).
-struct Geo {
+template <class T>
+struct GeoT {
   int D;
   int nper;
-  const float* mn;
-  const float* mx;
-  const float* side;
+  const T* mn;
+  const T* mx;
+  const T* side;
   const int32_t* wrapped;
 };
+using Geo = GeoT<float>;
 
-__device__ __forceinline__ float normalize(const Geo& g, float c, int d) {
-  const float range = g.mx[d] - g.mn[d];
-  float r = c - static_cast<float>(cvt_trunc(c / range)) * range;
+template <class T>
+__device__ __forceinline__ T normalize(const GeoT<T>& g, T c, int d) {
+  const T range = g.mx[d] - g.mn[d];
+  T r = c - static_cast<T>(cvt_trunc(c / range)) * range;
   if (r >= g.mx[d])
     r -= range;
   else if (r < g.mn[d])
@@ -144,7 +218,8 @@ __device__ __forceinline__ float normalize(const Geo& g, float c, int d) {
   return r;
 }
 
-__device__ __forceinline__ int get_bin(const Geo& g, float c, int d) {
+template <class T>
+__device__ __forceinline__ int get_bin(const GeoT<T>& g, T c, int d) {
   int b;
   if (g.wrapped[d])
     b = cvt_trunc((normalize(g, c, d) - g.mn[d]) / g.side[d]);
@@ -155,13 +230,20 @@ __device__ __forceinline__ int get_bin(const Geo& g, float c, int d) {
   return b;
 }
 
-// Ordered-integer encoding of floats, so that integer min/max equal float min/max.
+// Ordered-integer encodings, so that integer min/max equal floating min/max.
 __device__ __forceinline__ int32_t float_to_ordered(float f) {
   const int32_t b = __float_as_int(f);
   return b >= 0 ? b : (b ^ 0x7fffffff);
 }
 __device__ __forceinline__ float ordered_to_float(int32_t o) {
   return __int_as_float(o >= 0 ? o : (o ^ 0x7fffffff));
+}
+__device__ __forceinline__ long long float_to_ordered(double f) {
+  const long long b = __double_as_longlong(f);
+  return b >= 0 ? b : (b ^ 0x7fffffffffffffffll);
+}
+__device__ __forceinline__ double ordered_to_float(long long o) {
+  return __longlong_as_double(o >= 0 ? o : (o ^ 0x7fffffffffffffffll));
 }
 
 }  // namespace clue_dev

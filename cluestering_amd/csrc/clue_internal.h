@@ -20,27 +20,46 @@ void radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int6
 
 // ---- stages (clue_stages.hip) ----
 
-// Input columns and the sorted copies, all device pointers.
-struct PointArrays {
-  const float* x[kMaxDim];
-  const float* w;
-  const float* sigma[kMaxDim];
-  const float* u;
-  const uint32_t* tags;
+// Integer types of the same width as the point type: ordered encodings of
+// coordinates (extremes) and of densities (nearest-higher keys).
+template <class T>
+struct FloatBits;
+template <>
+struct FloatBits<float> {
+  using Int = int32_t;
+  using UInt = uint32_t;
+};
+template <>
+struct FloatBits<double> {
+  using Int = long long;  // the 64-bit type of the HIP atomics
+  using UInt = unsigned long long;
 };
 
-struct SortedArrays {
-  float* x[kMaxDim];
-  float* w;
-  float* fw;      // flat kernel only: flat value times weight
-  float* sigma[kMaxDim];
-  float* u;
+// Input columns and the sorted copies, all device pointers.
+template <class T>
+struct PointArraysT {
+  const T* x[kMaxDim];
+  const T* w;
+  const T* sigma[kMaxDim];
+  const T* u;
+  const uint32_t* tags;
+};
+using PointArrays = PointArraysT<float>;
+
+template <class T>
+struct SortedArraysT {
+  T* x[kMaxDim];
+  T* w;
+  T* fw;          // flat kernel only: flat value times weight
+  T* sigma[kMaxDim];
+  T* u;
   uint32_t* tag;  // user tags, or the original index when no tags are given
   int32_t* perm;  // sorted position -> original index
 };
 
 // Everything one batch run needs on the device.
-struct BatchBuffers {
+template <class T>
+struct BatchBuffersT {
   int32_t n_events;
   int32_t n_total;
   int32_t n_chunks;
@@ -49,8 +68,8 @@ struct BatchBuffers {
   int32_t key_bits;
   const EventDesc* events;
   const Chunk* chunks;
-  int32_t* ext;    // [E][D][2] ordered-int extremes
-  float* geo;      // [E][3][D] mn, mx, side
+  typename FloatBits<T>::Int* ext;  // [E][D][2] ordered-integer extremes
+  T* geo;                           // [E][3][D] mn, mx, side
   uint32_t* keys0;
   int32_t* vals0;
   uint32_t* keys1;
@@ -65,27 +84,30 @@ struct BatchBuffers {
   int32_t* qperm_nh;   // [n_total] nearest-higher query order (may alias qperm)
   int32_t* qfine;      // [n_total] Morton order within tiles (input of both)
   uint32_t* qkey;      // [n_total] packed query keys (Morton, class for dc, class for dm)
-  float* srho;
-  uint64_t* skey;      // (rho, tag) ordering key of every sorted point
+  T* srho;
+  uint64_t* skey;      // ordering key of every sorted point: float (rho, tag); double rho
   int32_t* snh;
-  float* sdelta;
+  T* sdelta;
   int32_t* flag;       // [n_total]
   int32_t* rank;       // [n_total]
   int32_t* n_seeds;    // [1]
   int32_t* seeds;      // [n_total]
-  SortedArrays sorted;
+  SortedArraysT<T> sorted;
   int64_t sorted_stride;  // column stride of sorted.x[] and sorted.sigma[] blocks
 };
+using BatchBuffers = BatchBuffersT<float>;
 
-struct ResultArrays {
+template <class T>
+struct ResultArraysT {
   int32_t* cluster;
   int32_t* is_seed;
-  float* rho;
+  T* rho;
   int32_t* nh;
-  float* delta;
+  T* delta;
   int32_t* event_of_cluster;  // may be NULL
   int32_t local_ids;          // number clusters per event from 0 (served events)
 };
+using ResultArrays = ResultArraysT<float>;
 
 // Stage boundaries reported by the profiler.
 enum Stage {
@@ -107,7 +129,9 @@ struct StageHook {
 
 // Launches the whole pipeline for one batch on stream s. No host synchronisation,
 // no allocation, no memset: safe to capture in a hipGraph.
-void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuffers& bb,
-                     const ResultArrays& out, hipStream_t s, const StageHook* hook);
+// Instantiated for float and double.
+template <class T>
+void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const BatchBuffersT<T>& bb,
+                     const ResultArraysT<T>& out, hipStream_t s, const StageHook* hook);
 
 }  // namespace clue_dev

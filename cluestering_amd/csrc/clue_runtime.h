@@ -81,22 +81,43 @@ struct BatchPlan {
   clue_tuning tuning{};
 };
 
+// The C structs of each precision and the point type they carry.
+template <class P>
+struct ValueOf;
+template <>
+struct ValueOf<clue_params> {
+  using type = float;
+};
+template <>
+struct ValueOf<clue_params_f64> {
+  using type = double;
+};
+template <class P>
+using value_t = typename ValueOf<P>::type;
+
 // Tuning of a batch: `fixed` when given, else the built-in table's entry for the
 // batch's ndim and mean event size.
 clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events);
-clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t n_events,
-                            BatchPlan* bp, const clue_tuning* fixed = nullptr);
-// Largest float t with sqrtf(t) <= r.
-float sq_max(float r);
-clue_status validate(const clue_params* p);
-RunParams make_run_params(const clue_params* p, const clue_points* pts, const BatchPlan& bp);
-clue_status check_points(const clue_params* p, const clue_points* pts, int64_t n_expected);
-PointArrays point_arrays(const clue_params* p, const clue_points* pts);
+template <class P>
+clue_status make_batch_plan(const P* p, const uint32_t* sizes, int32_t n_events, BatchPlan* bp,
+                            const clue_tuning* fixed = nullptr);
+// Largest t with sqrt(t) <= r (correctly rounded sqrt, in the precision of r).
+template <class T>
+T sq_max(T r);
+template <class P>
+clue_status validate(const P* p);
+template <class P, class Pts>
+RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPlan& bp);
+template <class P, class Pts>
+clue_status check_points(const P* p, const Pts* pts, int64_t n_expected);
+template <class P, class Pts>
+PointArraysT<value_t<P>> point_arrays(const P* p, const Pts* pts);
 // Reserves the workspace of one batch, uploads its descriptors on stream s and
 // fills the BatchBuffers and result pointers (NULL results use workspace buffers).
-clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params* p,
-                          const clue_points* pts, clue_results* res, hipStream_t s,
-                          BatchBuffers* bb, ResultArrays* ra);
+template <class P, class Pts, class Res>
+clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
+                          Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
+                          ResultArraysT<value_t<P>>* ra);
 
 struct ScopedDevice {
   int prev = 0;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -40,8 +41,9 @@ clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events) {
   return out;
 }
 
-clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t n_events,
-                            BatchPlan* bp, const clue_tuning* fixed) {
+template <class P>
+clue_status make_batch_plan(const P* p, const uint32_t* sizes, int32_t n_events, BatchPlan* bp,
+                            const clue_tuning* fixed) {
   const int D = p->ndim;
   const int ppt = p->points_per_tile > 0 ? p->points_per_tile : kDefaultPointsPerTile;
   bp->events.clear();
@@ -92,41 +94,43 @@ clue_status make_batch_plan(const clue_params* p, const uint32_t* sizes, int32_t
   return CLUE_OK;
 }
 
-// Largest float t with sqrtf(t) <= r (sqrtf is correctly rounded on the host and on
-// gfx950), so that sqrt(t) <= r  <=>  t <= sq_max(r) for every float t.
-float sq_max(float r) {
-  if (!(r > 0.0f)) return -1.0f;
-  if (std::isinf(r)) return INFINITY;
-  float t = r * r;
-  if (std::isinf(t)) t = FLT_MAX;
-  while (t > 0.0f && std::sqrt(t) > r) t = std::nextafter(t, 0.0f);
+// Largest t with sqrt(t) <= r (sqrt is correctly rounded on the host and on
+// gfx950 in both precisions), so that sqrt(t) <= r  <=>  t <= sq_max(r) for every t.
+template <class T>
+T sq_max(T r) {
+  if (!(r > T(0))) return T(-1);
+  if (std::isinf(r)) return std::numeric_limits<T>::infinity();
+  T t = r * r;
+  if (std::isinf(t)) t = std::numeric_limits<T>::max();
+  while (t > T(0) && std::sqrt(t) > r) t = std::nextafter(t, T(0));
   while (true) {
-    const float nx = std::nextafter(t, INFINITY);
+    const T nx = std::nextafter(t, std::numeric_limits<T>::infinity());
     if (std::isinf(nx) || std::sqrt(nx) > r) break;
     t = nx;
   }
   return t;
 }
 
-clue_status validate(const clue_params* p) {
+template <class P>
+clue_status validate(const P* p) {
   if (!p) return fail(CLUE_ERR_INVALID_ARGUMENT, "null params");
   if (p->ndim < 1 || p->ndim > kMaxDim)
     return fail(CLUE_ERR_OUT_OF_RANGE, "This library only works up to 10 dimensions");
-  if (!(p->dc > 0.0f) || !(p->rhoc >= 0.0f) || !(p->dm > 0.0f) || !(p->seed_dc > 0.0f))
+  if (!(p->dc > 0) || !(p->rhoc >= 0) || !(p->dm > 0) || !(p->seed_dc > 0))
     return fail(CLUE_ERR_INVALID_ARGUMENT,
                 "Invalid clustering parameters. The parameters must be positive.");
   switch (p->kernel) {
     case CLUE_KERNEL_FLAT:
-      if (!(p->kparam[0] > 0.0f))
+      if (!(p->kparam[0] > 0))
         return fail(CLUE_ERR_INVALID_ARGUMENT, "Flat kernel parameter must be positive.");
       break;
     case CLUE_KERNEL_EXP:
-      if (!(p->kparam[0] > 0.0f) || !(p->kparam[1] > 0.0f))
+      if (!(p->kparam[0] > 0) || !(p->kparam[1] > 0))
         return fail(CLUE_ERR_INVALID_ARGUMENT,
                     "Exponential kernel parameters must be positive.");
       break;
     case CLUE_KERNEL_GAUSS:
-      if (!(p->kparam[0] > 0.0f) || !(p->kparam[1] > 0.0f))
+      if (!(p->kparam[0] > 0) || !(p->kparam[1] > 0))
         return fail(CLUE_ERR_INVALID_ARGUMENT, "Gaussian kernel parameters must be positive.");
       break;
     default:
@@ -138,10 +142,12 @@ clue_status validate(const clue_params* p) {
   return CLUE_OK;
 }
 
-RunParams make_run_params(const clue_params* p, const clue_points* pts, const BatchPlan& bp) {
+template <class P, class Pts>
+RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPlan& bp) {
+  using T = value_t<P>;
   const int n_events = static_cast<int>(bp.events.size());
   const int64_t n_total = bp.n_total;
-  RunParams rp{};
+  RunParamsT<T> rp{};
   rp.xcd_chunk_log2 = bp.tuning.xcd_chunk_log2;
   rp.block_waves = bp.tuning.block_waves > 0 ? bp.tuning.block_waves : 4;
   rp.D = p->ndim;
@@ -162,13 +168,14 @@ RunParams make_run_params(const clue_params* p, const clue_points* pts, const Ba
   rp.kp0 = p->kparam[0];
   rp.kp1 = p->kparam[1];
   // 2 * sigma * sigma, evaluated left to right (ConvolutionalKernel.hpp:40).
-  rp.gauss_den = (2.0f * p->kparam[0]) * p->kparam[0];
+  rp.gauss_den = (T(2) * p->kparam[0]) * p->kparam[0];
   {
     int ex = 0;
-    const float m = std::frexp(rp.gauss_den, &ex);
+    const T m = std::frexp(rp.gauss_den, &ex);
     // Power of two with a normal, finite reciprocal.
-    rp.gauss_pow2 = (m == 0.5f && rp.gauss_den >= 0x1p-120f && rp.gauss_den <= 0x1p120f) ? 1 : 0;
-    rp.gauss_inv_den = rp.gauss_pow2 ? std::ldexp(1.0f, 1 - ex) : 0.0f;
+    const T lim = sizeof(T) == 4 ? T(0x1p120) : T(0x1p1000);
+    rp.gauss_pow2 = (m == T(0.5) && rp.gauss_den >= T(1) / lim && rp.gauss_den <= lim) ? 1 : 0;
+    rp.gauss_inv_den = rp.gauss_pow2 ? std::ldexp(T(1), 1 - ex) : T(0);
   }
   // Within the radius the kernel's exp argument is -lambda*d (exp) or -d*d/(2s^2)
   // (gauss), with 0 <= d <= dc; bound its magnitude with a 1% margin for rounding.
@@ -181,18 +188,19 @@ RunParams make_run_params(const clue_params* p, const clue_points* pts, const Ba
   }
   for (int d = 0; d < kMaxDim; ++d) {
     rp.wrapped[d] = d < p->ndim ? (p->wrapped[d] ? 1 : 0) : 0;
-    rp.mparam[d] = d < p->ndim ? p->mparam[d] : 0.0f;
+    rp.mparam[d] = d < p->ndim ? p->mparam[d] : T(0);
   }
   // Weighted Euclidean with unit weights evaluates (1 * dx) * dx == dx * dx.
   if (rp.metric == CLUE_METRIC_WEIGHTED_EUCLIDEAN) {
     bool unit = true;
-    for (int d = 0; d < p->ndim; ++d) unit = unit && p->mparam[d] == 1.0f;
+    for (int d = 0; d < p->ndim; ++d) unit = unit && p->mparam[d] == 1;
     if (unit) rp.metric = CLUE_METRIC_EUCLIDEAN;
   }
   return rp;
 }
 
-clue_status check_points(const clue_params* p, const clue_points* pts, int64_t n_expected) {
+template <class P, class Pts>
+clue_status check_points(const P* p, const Pts* pts, int64_t n_expected) {
   if (!pts) return fail(CLUE_ERR_INVALID_ARGUMENT, "null points");
   if (pts->n < 1 || (n_expected >= 0 && pts->n != n_expected))
     return fail(CLUE_ERR_INVALID_ARGUMENT, "the number of points must be positive");
@@ -211,17 +219,20 @@ clue_status check_points(const clue_params* p, const clue_points* pts, int64_t n
 
 // Reserves the workspace of one batch and fills the BatchBuffers. Descriptor
 // uploads are enqueued on the stream (they precede any kernel that reads them).
-clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params* p,
-                          const clue_points* pts, clue_results* res, hipStream_t s,
-                          BatchBuffers* bb, ResultArrays* ra) {
+template <class P, class Pts, class Res>
+clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
+                          Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
+                          ResultArraysT<value_t<P>>* ra) {
+  using T = value_t<P>;
+  using I = typename FloatBits<T>::Int;
   const int D = p->ndim;
   const int64_t n = bp.n_total;
   const int64_t E = static_cast<int64_t>(bp.events.size());
   const int64_t C = static_cast<int64_t>(bp.chunks.size());
-  const int64_t T = bp.n_tiles_total;
+  const int64_t NT = bp.n_tiles_total;
   const int64_t hist = radix_hist_elems(n);
   const int64_t scan_elems =
-      std::max({scan_temp_elems(n), scan_temp_elems(hist), scan_temp_elems(T)}) + 16;
+      std::max({scan_temp_elems(n), scan_temp_elems(hist), scan_temp_elems(NT)}) + 16;
   const int64_t stride = (n + 63) / 64 * 64;
   const bool has_sigma = pts->sigma[0] != nullptr;
   struct Req {
@@ -231,38 +242,38 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   const Req reqs[] = {
       {S_EVENTS, sizeof(EventDesc) * E},
       {S_CHUNKS, sizeof(Chunk) * C},
-      {S_EXT, sizeof(int32_t) * E * D * 2},
-      {S_GEO, sizeof(float) * E * 3 * D},
+      {S_EXT, sizeof(I) * E * D * 2},
+      {S_GEO, sizeof(T) * E * 3 * D},
       {S_K0, sizeof(uint32_t) * n},
       {S_V0, sizeof(int32_t) * n},
       {S_K1, sizeof(uint32_t) * n},
       {S_V1, sizeof(int32_t) * n},
       {S_HIST, sizeof(int32_t) * hist},
       {S_SCAN, sizeof(int32_t) * scan_elems},
-      {S_TOFF, sizeof(int32_t) * (T + 1)},
-      {S_ICOUNT, sizeof(int32_t) * T},
+      {S_TOFF, sizeof(int32_t) * (NT + 1)},
+      {S_ICOUNT, sizeof(int32_t) * NT},
       {S_NITEMS, sizeof(int32_t) * 4},
       {S_ITEMS, sizeof(Item) * bp.max_items},
       {S_QPERM, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc) ? sizeof(int32_t) * n : 0},
       {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_QKEY, bp.tuning.group_queries ? sizeof(uint32_t) * n : 0},
-      {S_SRHO, sizeof(float) * n},
+      {S_SRHO, sizeof(T) * n},
       {S_SKEY, sizeof(uint64_t) * n},
-      {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(float) * n : 0},
+      {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(T) * n : 0},
       {S_SNH, sizeof(int32_t) * n},
-      {S_SDELTA, sizeof(float) * n},
+      {S_SDELTA, sizeof(T) * n},
       {S_FLAG, sizeof(int32_t) * n},
       {S_RANK, sizeof(int32_t) * n},
       {S_NSEEDS, sizeof(int32_t) * 4},
       {S_SEEDS, sizeof(int32_t) * n},
-      {S_SX, sizeof(float) * stride * D},
-      {S_SW, sizeof(float) * n},
-      {S_SSIG, has_sigma ? sizeof(float) * stride * D : 0},
-      {S_SU, pts->density_uncertainty ? sizeof(float) * n : 0},
+      {S_SX, sizeof(T) * stride * D},
+      {S_SW, sizeof(T) * n},
+      {S_SSIG, has_sigma ? sizeof(T) * stride * D : 0},
+      {S_SU, pts->density_uncertainty ? sizeof(T) * n : 0},
       {S_STAG, sizeof(uint32_t) * n},
       {S_PERM, sizeof(int32_t) * n},
-      {S_RHO, res->rho ? 0 : sizeof(float) * n},
+      {S_RHO, res->rho ? 0 : sizeof(T) * n},
       {S_NH, res->nearest_higher ? 0 : sizeof(int32_t) * n},
       {S_ISSEED, res->is_seed ? 0 : sizeof(int32_t) * n},
   };
@@ -271,18 +282,18 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
                           hipMemcpyHostToDevice, s));
   CLUE_HIP(hipMemcpyAsync(ws.buf[S_CHUNKS].p, bp.chunks.data(), sizeof(Chunk) * C,
                           hipMemcpyHostToDevice, s));
-  BatchBuffers& b = *bb;
-  b = BatchBuffers{};
+  auto& b = *bb;
+  b = BatchBuffersT<T>{};
   b.n_events = static_cast<int32_t>(E);
   b.n_total = static_cast<int32_t>(n);
   b.n_chunks = static_cast<int32_t>(C);
-  b.n_tiles_total = static_cast<int32_t>(T);
+  b.n_tiles_total = static_cast<int32_t>(NT);
   b.max_items = static_cast<int32_t>(bp.max_items);
   b.key_bits = bp.key_bits;
   b.events = ws.buf[S_EVENTS].as<EventDesc>();
   b.chunks = ws.buf[S_CHUNKS].as<Chunk>();
-  b.ext = ws.buf[S_EXT].as<int32_t>();
-  b.geo = ws.buf[S_GEO].as<float>();
+  b.ext = ws.buf[S_EXT].as<I>();
+  b.geo = ws.buf[S_GEO].as<T>();
   b.keys0 = ws.buf[S_K0].as<uint32_t>();
   b.vals0 = ws.buf[S_V0].as<int32_t>();
   b.keys1 = ws.buf[S_K1].as<uint32_t>();
@@ -298,27 +309,27 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
                                                           : b.qperm;
   b.qfine = bp.tuning.group_queries ? ws.buf[S_QFINE].as<int32_t>() : nullptr;
   b.qkey = bp.tuning.group_queries ? ws.buf[S_QKEY].as<uint32_t>() : nullptr;
-  b.srho = ws.buf[S_SRHO].as<float>();
+  b.srho = ws.buf[S_SRHO].as<T>();
   b.skey = ws.buf[S_SKEY].as<uint64_t>();
   b.snh = ws.buf[S_SNH].as<int32_t>();
-  b.sdelta = ws.buf[S_SDELTA].as<float>();
+  b.sdelta = ws.buf[S_SDELTA].as<T>();
   b.flag = ws.buf[S_FLAG].as<int32_t>();
   b.rank = ws.buf[S_RANK].as<int32_t>();
   b.n_seeds = ws.buf[S_NSEEDS].as<int32_t>();
   b.seeds = ws.buf[S_SEEDS].as<int32_t>();
   b.sorted_stride = stride;
   for (int d = 0; d < kMaxDim; ++d) {
-    b.sorted.x[d] = d < D ? ws.buf[S_SX].as<float>() + stride * d : nullptr;
-    b.sorted.sigma[d] = (has_sigma && d < D) ? ws.buf[S_SSIG].as<float>() + stride * d : nullptr;
+    b.sorted.x[d] = d < D ? ws.buf[S_SX].as<T>() + stride * d : nullptr;
+    b.sorted.sigma[d] = (has_sigma && d < D) ? ws.buf[S_SSIG].as<T>() + stride * d : nullptr;
   }
-  b.sorted.w = ws.buf[S_SW].as<float>();
-  b.sorted.fw = p->kernel == CLUE_KERNEL_FLAT ? ws.buf[S_SFW].as<float>() : nullptr;
-  b.sorted.u = pts->density_uncertainty ? ws.buf[S_SU].as<float>() : nullptr;
+  b.sorted.w = ws.buf[S_SW].as<T>();
+  b.sorted.fw = p->kernel == CLUE_KERNEL_FLAT ? ws.buf[S_SFW].as<T>() : nullptr;
+  b.sorted.u = pts->density_uncertainty ? ws.buf[S_SU].as<T>() : nullptr;
   b.sorted.tag = ws.buf[S_STAG].as<uint32_t>();
   b.sorted.perm = ws.buf[S_PERM].as<int32_t>();
   ra->cluster = res->cluster_index;
   ra->is_seed = res->is_seed ? res->is_seed : ws.buf[S_ISSEED].as<int32_t>();
-  ra->rho = res->rho ? res->rho : ws.buf[S_RHO].as<float>();
+  ra->rho = res->rho ? res->rho : ws.buf[S_RHO].as<T>();
   ra->nh = res->nearest_higher ? res->nearest_higher : ws.buf[S_NH].as<int32_t>();
   ra->delta = res->delta;
   ra->event_of_cluster = nullptr;
@@ -326,8 +337,9 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const clue_params*
   return CLUE_OK;
 }
 
-PointArrays point_arrays(const clue_params* p, const clue_points* pts) {
-  PointArrays a{};
+template <class P, class Pts>
+PointArraysT<value_t<P>> point_arrays(const P* p, const Pts* pts) {
+  PointArraysT<value_t<P>> a{};
   for (int d = 0; d < kMaxDim; ++d) {
     a.x[d] = d < p->ndim ? pts->coords[d] : nullptr;
     a.sigma[d] = d < p->ndim ? pts->sigma[d] : nullptr;
@@ -338,5 +350,28 @@ PointArrays point_arrays(const clue_params* p, const clue_points* pts) {
   return a;
 }
 
+
+template clue_status make_batch_plan<clue_params>(const clue_params*, const uint32_t*, int32_t,
+                                                  BatchPlan*, const clue_tuning*);
+template clue_status make_batch_plan<clue_params_f64>(const clue_params_f64*, const uint32_t*,
+                                                      int32_t, BatchPlan*, const clue_tuning*);
+template float sq_max<float>(float);
+template double sq_max<double>(double);
+template clue_status validate<clue_params>(const clue_params*);
+template clue_status validate<clue_params_f64>(const clue_params_f64*);
+template RunParamsT<float> make_run_params(const clue_params*, const clue_points*,
+                                           const BatchPlan&);
+template RunParamsT<double> make_run_params(const clue_params_f64*, const clue_points_f64*,
+                                            const BatchPlan&);
+template clue_status check_points(const clue_params*, const clue_points*, int64_t);
+template clue_status check_points(const clue_params_f64*, const clue_points_f64*, int64_t);
+template PointArraysT<float> point_arrays(const clue_params*, const clue_points*);
+template PointArraysT<double> point_arrays(const clue_params_f64*, const clue_points_f64*);
+template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params*,
+                                   const clue_points*, clue_results*, hipStream_t,
+                                   BatchBuffersT<float>*, ResultArraysT<float>*);
+template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params_f64*,
+                                   const clue_points_f64*, clue_results_f64*, hipStream_t,
+                                   BatchBuffersT<double>*, ResultArraysT<double>*);
 
 }  // namespace clue_rt

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <limits>
 #include <type_traits>
 
 #include "clue_device.h"
@@ -30,7 +31,6 @@ constexpr int kWaves = kBlock / kWave;
 // workgroup runs on one CU, so its consecutive work items (neighbouring tiles)
 // share that CU's scalar cache, while smaller workgroups spread more evenly.
 constexpr int kMaxNeighbourBlock = 1024;
-constexpr float kSqrt2 = 0x1.6a09e6p+0f;  // sqrtf(2.0f), correctly rounded
 
 enum { M_EUCLID = 0, M_WEUCLID = 1, M_PERIODIC = 2, M_MANHATTAN = 3, M_CHEBYSHEV = 4,
        M_WCHEBYSHEV = 5, M_MAHALANOBIS = 6 };
@@ -67,24 +67,28 @@ struct MetricTraits {
 // (a1) Per-event coordinate extremes (ComputeTiles.hpp:17-36). NaN coordinates
 // are skipped, as the reference's std::max/std::min fold skips them.
 // ---------------------------------------------------------------------------
+template <class T>
 struct CoordPtrs {
-  const float* x[kMaxDim];
+  const T* x[kMaxDim];
 };
 
-__global__ __launch_bounds__(kBlock) void k_extremes(CoordPtrs c, int D,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_extremes(CoordPtrs<T> c, int D,
                                                      const Chunk* __restrict__ chunks,
-                                                     int32_t* __restrict__ ext) {
+                                                     typename FloatBits<T>::Int* __restrict__ ext) {
+  using I = typename FloatBits<T>::Int;
+  constexpr I kHi = std::numeric_limits<I>::max(), kLo = std::numeric_limits<I>::min();
   const Chunk ch = chunks[blockIdx.x];
-  __shared__ int32_t smin[kWaves][kMaxDim], smax[kWaves][kMaxDim];
+  __shared__ I smin[kWaves][kMaxDim], smax[kWaves][kMaxDim];
   const int w = threadIdx.x / kWave;
   for (int d = 0; d < D; ++d) {
-    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    I lo = kHi, hi = kLo;
     for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
-      const float v = c.x[d][i];
+      const T v = c.x[d][i];
       if (v != v) continue;
-      const int32_t o = float_to_ordered(v);
-      lo = min(lo, o);
-      hi = max(hi, o);
+      const I o = float_to_ordered(v);
+      lo = o < lo ? o : lo;
+      hi = o > hi ? o : hi;
     }
     lo = wave_min_i(lo);
     hi = wave_max_i(hi);
@@ -96,45 +100,53 @@ __global__ __launch_bounds__(kBlock) void k_extremes(CoordPtrs c, int D,
   __syncthreads();
   if (threadIdx.x < D) {
     const int d = threadIdx.x;
-    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    I lo = kHi, hi = kLo;
     for (int k = 0; k < kWaves; ++k) {
-      lo = min(lo, smin[k][d]);
-      hi = max(hi, smax[k][d]);
+      lo = smin[k][d] < lo ? smin[k][d] : lo;
+      hi = smax[k][d] > hi ? smax[k][d] : hi;
     }
-    if (lo != INT32_MAX) atomicMin(&ext[(ch.event * D + d) * 2 + 0], lo);
-    if (hi != INT32_MIN) atomicMax(&ext[(ch.event * D + d) * 2 + 1], hi);
+    if (lo != kHi) atomicMin(&ext[(ch.event * D + d) * 2 + 0], lo);
+    if (hi != kLo) atomicMax(&ext[(ch.event * D + d) * 2 + 1], hi);
   }
 }
 
-// Extremes start from the identities of min (INT32_MAX) and max (INT32_MIN).
-__global__ void k_init_extremes(int32_t* __restrict__ ext, int64_t pairs) {
+// Extremes start from the identities of min and max.
+template <class I>
+__global__ void k_init_extremes(I* __restrict__ ext, int64_t pairs) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k < pairs) {
-    ext[2 * k] = INT32_MAX;
-    ext[2 * k + 1] = INT32_MIN;
+    ext[2 * k] = std::numeric_limits<I>::max();
+    ext[2 * k + 1] = std::numeric_limits<I>::min();
   }
 }
 
 // geo[e] = {mn[D], mx[D], side[D]}, side = (max - min) / nper (ComputeTiles.hpp:33-34).
-__global__ void k_geometry(const int32_t* __restrict__ ext, const EventDesc* __restrict__ events,
-                           int n_events, int D, float* __restrict__ geo) {
+template <class T>
+__global__ void k_geometry(const typename FloatBits<T>::Int* __restrict__ ext,
+                           const EventDesc* __restrict__ events, int n_events, int D,
+                           T* __restrict__ geo) {
+  using I = typename FloatBits<T>::Int;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_events) return;
   const int nper = events[e].nper;
   for (int d = 0; d < D; ++d) {
-    const int32_t lo = ext[(e * D + d) * 2 + 0];
-    const int32_t hi = ext[(e * D + d) * 2 + 1];
-    const float mn = lo == INT32_MAX ? FLT_MAX : ordered_to_float(lo);
-    const float mx = hi == INT32_MIN ? -FLT_MAX : ordered_to_float(hi);
-    float* g = geo + static_cast<int64_t>(e) * 3 * D;
+    const I lo = ext[(e * D + d) * 2 + 0];
+    const I hi = ext[(e * D + d) * 2 + 1];
+    const T mn = lo == std::numeric_limits<I>::max() ? std::numeric_limits<T>::max()
+                                                     : ordered_to_float(lo);
+    const T mx = hi == std::numeric_limits<I>::min() ? std::numeric_limits<T>::lowest()
+                                                     : ordered_to_float(hi);
+    T* g = geo + static_cast<int64_t>(e) * 3 * D;
     g[d] = mn;
     g[D + d] = mx;
-    g[2 * D + d] = (mx - mn) / static_cast<float>(nper);
+    g[2 * D + d] = (mx - mn) / static_cast<T>(nper);
   }
 }
 
-__device__ __forceinline__ Geo event_geo(const float* geo, const RunParams& rp, int e, int nper) {
-  Geo g;
+template <class T>
+__device__ __forceinline__ GeoT<T> event_geo(const T* geo, const RunParamsT<T>& rp, int e,
+                                             int nper) {
+  GeoT<T> g;
   g.D = rp.D;
   g.nper = nper;
   g.mn = geo + static_cast<int64_t>(e) * 3 * rp.D;
@@ -148,15 +160,16 @@ __device__ __forceinline__ Geo event_geo(const float* geo, const RunParams& rp, 
 // (a2) Tile key of every point: t_base + sum_d nper^(D-1-d) * bin_d
 // (TilesView.hpp:54-64), and the identity permutation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_tile_keys(CoordPtrs c, const RunParams rp,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_tile_keys(CoordPtrs<T> c, const RunParamsT<T> rp,
                                                       const Chunk* __restrict__ chunks,
                                                       const EventDesc* __restrict__ events,
-                                                      const float* __restrict__ geo,
+                                                      const T* __restrict__ geo,
                                                       uint32_t* __restrict__ keys,
                                                       int32_t* __restrict__ vals) {
   const Chunk ch = chunks[blockIdx.x];
   const EventDesc ev = events[ch.event];
-  const Geo g = event_geo(geo, rp, ch.event, ev.nper);
+  const GeoT<T> g = event_geo(geo, rp, ch.event, ev.nper);
   for (int i = ch.begin + threadIdx.x; i < ch.end; i += kBlock) {
     int t = 0;
     for (int d = 0; d < rp.D; ++d) t = t * ev.nper + get_bin(g, c.x[d][i], d);
@@ -169,13 +182,15 @@ __global__ __launch_bounds__(kBlock) void k_tile_keys(CoordPtrs c, const RunPara
 // (a3) Gather the tile-sorted SoA copy and the tile offsets. toff[k] is the first
 // sorted position whose key is >= k, so tile k spans [toff[k], toff[k+1]).
 // ---------------------------------------------------------------------------
+template <class T>
 struct GatherArgs {
-  PointArrays in;
-  SortedArrays out;
+  PointArraysT<T> in;
+  SortedArraysT<T> out;
   int64_t cs;  // column stride of the sorted coordinate block
 };
 
-__global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs a, const RunParams rp,
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs<T> a, const RunParamsT<T> rp,
                                                    const uint32_t* __restrict__ keys,
                                                    const int32_t* __restrict__ perm, int n,
                                                    int n_tiles_total, int32_t* __restrict__ toff) {
@@ -183,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs a, const Run
   if (s >= n) return;
   const int i = perm[s];
   for (int d = 0; d < rp.D; ++d) a.out.x[d][s] = a.in.x[d][i];
-  const float wi = a.in.w[i];
+  const T wi = a.in.w[i];
   a.out.w[s] = wi;
   if (a.out.fw) a.out.fw[s] = rp.kp0 * wi;  // flat kernel: K * w for j != i
   if (rp.has_sigma)
@@ -237,75 +252,74 @@ __global__ void k_item_fill(const int32_t* __restrict__ toff, const int32_t* __r
 // (internal/meta/accumulate.hpp): t0 + (t1 + (... + (t_{D-1} + 0))).
 // For square-root metrics the value returned is the sum under the root.
 // ---------------------------------------------------------------------------
-template <int D, int M>
-__device__ __forceinline__ float metric_value(const float (&xi)[D], const float (&si)[D],
-                                              const float* __restrict__ sx, int64_t cs, int j,
-                                              const float* __restrict__ ssig,
-                                              const RunParams& rp) {
+template <int D, int M, class T>
+__device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
+                                          const T* __restrict__ sx, int64_t cs, int j,
+                                          const T* __restrict__ ssig, const RunParamsT<T>& rp) {
   if constexpr (M == M_EUCLID) {
-    float s;
+    T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const float dx = xi[d] - sx[d * cs + j];
+      const T dx = xi[d] - sx[d * cs + j];
       s = (d == D - 1) ? dx * dx : dx * dx + s;  // t + 0 == t for t = dx*dx >= +0
     }
     return s;
   } else if constexpr (M == M_WEUCLID) {
-    float s = 0.0f;
+    T s = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const float dx = xi[d] - sx[d * cs + j];
+      const T dx = xi[d] - sx[d * cs + j];
       s = rp.mparam[d] * dx * dx + s;
     }
     return s;
   } else if constexpr (M == M_PERIODIC) {
-    float s;
+    T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const float a = fabsf(xi[d] - sx[d * cs + j]);
-      const float pa = rp.mparam[d] - a;
-      const float m = (pa < a) ? pa : a;  // std::min(a, P - a)
+      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
+      const T pa = rp.mparam[d] - a;
+      const T m = (pa < a) ? pa : a;  // std::min(a, P - a)
       s = (d == D - 1) ? m * m : m * m + s;
     }
     return s;
   } else if constexpr (M == M_MANHATTAN) {
-    float s;
+    T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const float a = fabsf(xi[d] - sx[d * cs + j]);
+      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
       s = (d == D - 1) ? a : a + s;
     }
     return s;
   } else if constexpr (M == M_CHEBYSHEV) {
-    float m = 0.0f;
+    T m = T(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const float a = fabsf(xi[d] - sx[d * cs + j]);
+      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
       m = (m < a) ? a : m;  // std::max(m, a)
     }
     return m;
   } else if constexpr (M == M_WCHEBYSHEV) {
-    float m = 0.0f;
+    T m = T(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const float a = rp.mparam[d] * fabsf(xi[d] - sx[d * cs + j]);
+      const T a = rp.mparam[d] * __builtin_fabs(xi[d] - sx[d * cs + j]);
       m = (m < a) ? a : m;
     }
     return m;
   } else {  // M_MAHALANOBIS: (dx*dx) / (si*si + sj*sj)
-    float s = 0.0f;
+    T s = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const float dx = xi[d] - sx[d * cs + j];
-      const float sj = ssig[d * cs + j];
+      const T dx = xi[d] - sx[d * cs + j];
+      const T sj = ssig[d * cs + j];
       s = dx * dx / (si[d] * si[d] + sj * sj) + s;
     }
     return s;
   }
 }
 
-template <int M>
-__device__ __forceinline__ float finish_distance(float v) {
+template <int M, class T>
+__device__ __forceinline__ T finish_distance(T v) {
   if constexpr (MetricTraits<M>::kSqrt)
     return sqrt_rn(v);
   else
@@ -313,14 +327,25 @@ __device__ __forceinline__ float finish_distance(float v) {
 }
 
 // Search box of one point (ClusteringKernels.hpp:111-121, TilesView.hpp:77-87).
-template <int D>
-__device__ __forceinline__ void search_box(const Geo& g, const float (&xi)[D],
-                                           const float (&si)[D], float R, int (&lo)[D],
-                                           int (&hi)[D]) {
+// sqrt(2) correctly rounded in each precision (std::sqrt of 2 in the reference).
+template <class T>
+struct Sqrt2;
+template <>
+struct Sqrt2<float> {
+  static constexpr float v = 0x1.6a09e6p+0f;
+};
+template <>
+struct Sqrt2<double> {
+  static constexpr double v = 0x1.6a09e667f3bcdp+0;
+};
+
+template <int D, class T>
+__device__ __forceinline__ void search_box(const GeoT<T>& g, const T (&xi)[D], const T (&si)[D],
+                                           T R, int (&lo)[D], int (&hi)[D]) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const float rs = R * si[d] * kSqrt2;
-    const float r = (R < rs) ? rs : R;  // std::max(R, R * sigma * sqrt(2))
+    const T rs = R * si[d] * Sqrt2<T>::v;
+    const T r = (R < rs) ? rs : R;  // std::max(R, R * sigma * sqrt(2))
     int a = get_bin(g, xi[d] - r, d);
     int b = get_bin(g, xi[d] + r, d);
     if (g.wrapped[d] && a > b) b += g.nper;
@@ -380,10 +405,9 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
 // ---------------------------------------------------------------------------
 constexpr int kQueryDims = 3;  // dimensions that enter the keys
 
-template <int D>
-__device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[D],
-                                                const float (&si)[D], float R,
-                                                const int (&tb)[D]) {
+template <int D, class T>
+__device__ __forceinline__ uint32_t query_class(const GeoT<T>& g, const T (&xi)[D],
+                                                const T (&si)[D], T R, const int (&tb)[D]) {
   int lo[D], hi[D];
   search_box<D>(g, xi, si, R, lo, hi);
   uint32_t c = 0;
@@ -395,8 +419,8 @@ __device__ __forceinline__ uint32_t query_class(const Geo& g, const float (&xi)[
 
 // Morton index of the point's sub-cell: 8 bits over the first (up to) three
 // dimensions of the tile, bits interleaved from the most significant.
-template <int D>
-__device__ __forceinline__ uint32_t query_fine(const Geo& g, const float (&xi)[D],
+template <int D, class T>
+__device__ __forceinline__ uint32_t query_fine(const GeoT<T>& g, const T (&xi)[D],
                                                const int (&tb)[D]) {
   constexpr int kD = D < kQueryDims ? D : kQueryDims;
   constexpr int kB = 8 / kD;  // bits per dimension
@@ -404,9 +428,9 @@ __device__ __forceinline__ uint32_t query_fine(const Geo& g, const float (&xi)[D
   uint32_t q[kD];
 #pragma unroll
   for (int d = 0; d < kD; ++d) {
-    const float c = g.wrapped[d] ? normalize(g, xi[d], d) : xi[d];
-    const float f = ((c - g.mn[d]) / g.side[d] - static_cast<float>(tb[d])) * kS;
-    q[d] = (f >= 0.0f) ? (f < static_cast<float>(kS) ? static_cast<uint32_t>(f) : kS - 1u) : 0u;
+    const T c = g.wrapped[d] ? normalize(g, xi[d], d) : xi[d];
+    const T f = ((c - g.mn[d]) / g.side[d] - static_cast<T>(tb[d])) * static_cast<T>(kS);
+    q[d] = (f >= T(0)) ? (f < static_cast<T>(kS) ? static_cast<uint32_t>(f) : kS - 1u) : 0u;
   }
   uint32_t key = 0;
 #pragma unroll
@@ -478,11 +502,11 @@ __device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* 
 // then three stable counting sorts: sorted order -> Morton order (qfine), and
 // Morton order -> (class, Morton) for dc (qperm) and for dm (qperm_nh, skipped
 // when it aliases qperm).
-template <int D>
+template <int D, class T>
 __global__ __launch_bounds__(kBlock) void k_query_order(
-    const RunParams rp, const EventDesc* __restrict__ events, int n_events,
-    const float* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
-    const float* __restrict__ sx, int64_t cs, const float* __restrict__ ssig,
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, int n_events,
+    const T* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
+    const T* __restrict__ sx, int64_t cs, const T* __restrict__ ssig,
     uint32_t* __restrict__ qkey, int32_t* __restrict__ qfine, int32_t* __restrict__ qperm,
     int32_t* __restrict__ qperm_nh) {
   constexpr int kD = D < kQueryDims ? D : kQueryDims;
@@ -511,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       hi_e = mid - 1;
   }
   const EventDesc ev = events[lo_e];
-  const Geo g = event_geo(geo, rp, lo_e, ev.nper);
+  const GeoT<T> g = event_geo(geo, rp, lo_e, ev.nper);
   int tb[D];
   {
     int r = t - ev.t_base;
@@ -522,11 +546,11 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
     }
   }
   for (int s = b + lane; s < e; s += kWave) {
-    float xi[D], si[D];
+    T xi[D], si[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       xi[d] = sx[d * cs + s];
-      si[d] = rp.has_sigma ? ssig[d * cs + s] : 0.0f;
+      si[d] = rp.has_sigma ? ssig[d * cs + s] : T(0);
     }
     const uint32_t fine = query_fine<D>(g, xi, tb);
     const uint32_t c_dc = query_class<D>(g, xi, si, rp.dc, tb);
@@ -555,23 +579,35 @@ enum { K_GAUSS2 = 3 };
 // exp of an argument the host has shown to lie in [LO, HI] for every candidate
 // within the radius (rp.exp_in_range) skips the range handling; both forms give
 // the same value there.
-__device__ __forceinline__ float kernel_exp(float x, const RunParams& rp) {
+__device__ __forceinline__ float kernel_exp(float x, const RunParamsT<float>& rp) {
   return rp.exp_in_range ? expf_cw_core(x) : expf_cw(x);
 }
+__device__ __forceinline__ double kernel_exp(double x, const RunParamsT<double>& rp) {
+  return rp.exp_in_range ? expd_cw_core(x) : expd_cw(x);
+}
 
-template <int M, int K>
-__device__ __forceinline__ float kernel_value(float v, const RunParams& rp) {
+template <int M, int K, class T>
+__device__ __forceinline__ T kernel_value(T v, const RunParamsT<T>& rp) {
   if constexpr (K == K_FLAT) {
     return rp.kp0;
   } else if constexpr (K == K_EXP) {
     return rp.kp1 * kernel_exp(-rp.kp0 * finish_distance<M>(v), rp);
   } else if constexpr (K == K_GAUSS) {
-    const float dd = finish_distance<M>(v);
+    const T dd = finish_distance<M>(v);
     return rp.kp1 * kernel_exp(-dd * dd / rp.gauss_den, rp);
   } else {
-    const float dd = finish_distance<M>(v);
+    const T dd = finish_distance<M>(v);
     return rp.kp1 * kernel_exp((-dd * dd) * rp.gauss_inv_den, rp);
   }
+}
+
+// Ordering key stored with each density: float packs (rho, tag) into 64 bits;
+// double keeps rho's order alone and compares tags separately.
+__device__ __forceinline__ uint64_t density_key(float rho, uint32_t tag) {
+  return rho_tag_key(rho, tag, false);
+}
+__device__ __forceinline__ uint64_t density_key(double rho, uint32_t) {
+  return rho_key_f64(rho, false);
 }
 
 // Candidates are walked in blocks of U so that their wave-uniform data arrives
@@ -588,30 +624,30 @@ struct Unroll {
 // (ClusteringKernels.hpp:53-69). Candidates beyond dc add +0 and are skipped;
 // the additions keep the candidate order.
 // ---------------------------------------------------------------------------
-template <int D, int M, int K>
+template <int D, int M, int K, class T>
 __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
-    const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
-    const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
-    const float* __restrict__ sw, const float* __restrict__ sfw, const float* __restrict__ ssig,
+    const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
+    const T* __restrict__ sw, const T* __restrict__ sfw, const T* __restrict__ ssig,
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
-    float* __restrict__ srho, uint64_t* __restrict__ skey) {
+    T* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
   const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
                    threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
-  const Geo g = event_geo(geo, rp, it.event, ev.nper);
+  const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
   const int q = it.begin + lane_id();
   const bool active = q < it.end;
   const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
-  float xi[D], si[D];
+  T xi[D], si[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    xi[d] = active ? sx[d * cs + s] : 0.0f;
-    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+    xi[d] = active ? sx[d * cs + s] : T(0);
+    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
   }
   int lo[D], hi[D];
   search_box<D>(g, xi, si, rp.dc, lo, hi);
@@ -622,34 +658,34 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
       hi[d] = INT32_MIN;
     }
   }
-  const float thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
-  float rho = 0.0f;
+  const T thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
+  T rho = T(0);
   // The diagonal term K(i,i) * w_i = w_i; the flat kernel's off-diagonal weights
   // come pre-multiplied (fw = f * w).
-  const float wself = active ? sw[s] : 0.0f;
-  const float* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
+  const T wself = active ? sw[s] : T(0);
+  const T* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
 
   // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
   // wj is the candidate's column value, loaded with its block of candidates.
-  auto add = [&](float v, int j, float wj, float thr, auto own_c) {
+  auto add = [&](T v, int j, T wj, T thr, auto own_c) {
     constexpr bool own = decltype(own_c)::value;
     if constexpr (K == K_FLAT) {
       // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
       // the reference's addition of a zero term.
-      const float next = rho + ((own && j == s) ? wself : wj);
+      const T next = rho + ((own && j == s) ? wself : wj);
       rho = (v <= thr) ? next : rho;
     } else {
       if (v <= thr) {
-        const float kv = (own && j == s) ? 1.0f : kernel_value<M, K>(v, rp);
+        const T kv = (own && j == s) ? T(1) : kernel_value<M, K>(v, rp);
         rho += kv * wj;
       }
     }
   };
   // own_c: std::true_type for the point's own tile, where the diagonal term lies.
-  auto walk = [&](int j0, int j1, float thr, auto own_c) {
+  auto walk = [&](int j0, int j1, T thr, auto own_c) {
     int j = j0;
     for (; j + U <= j1; j += U) {
-      float v[U], w[U];
+      T v[U], w[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
 #pragma unroll
@@ -664,7 +700,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
     const int j0 = toff[t];
     const int j1 = toff[t + 1];
-    const float thr = in ? thr_in : -1.0f;
+    const T thr = in ? thr_in : T(-1);
     if (t == it.tile)
       walk(j0, j1, thr, std::true_type{});
     else
@@ -672,7 +708,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
   });
   if (active) {
     srho[s] = rho;
-    skey[s] = rho_tag_key(rho, stag[s], false);
+    skey[s] = density_key(rho, stag[s]);
   }
 }
 
@@ -696,40 +732,73 @@ __device__ __forceinline__ float prune_threshold(float eff_thr, float delta) {
     return delta < eff_thr ? delta : eff_thr;
   }
 }
+template <int M>
+__device__ __forceinline__ double prune_threshold(double eff_thr, double delta) {
+  if constexpr (MetricTraits<M>::kSqrt) {
+    // Double: t <= delta^2 (1 + 2^-51.9) follows; delta*delta*(1 + 2^-48) rounded
+    // bounds it from above for delta >= 2^-500.
+    if (!(delta >= 0x1p-500)) return eff_thr;
+    const double b = (delta * delta) * 0x1.000000000001p+0;
+    return b < eff_thr ? b : eff_thr;
+  } else {
+    return delta < eff_thr ? delta : eff_thr;
+  }
+}
 
-template <int D, int M>
+// "j is higher than i" (ClusteringKernels.hpp:180-182) from the stored keys. Float
+// keys hold (rho, tag); double keys hold rho's order only, so equal densities
+// compare the tags here (rho > 0 is the key above that of +0).
+__device__ __forceinline__ bool key_higher(uint64_t kj, uint32_t, uint64_t ki, uint32_t, float) {
+  return kj > ki;
+}
+__device__ __forceinline__ bool key_higher(uint64_t kj, uint32_t tj, uint64_t ki, uint32_t ti,
+                                           double) {
+  return (kj > ki) | ((kj == ki) & (kj > 0x8000000000000000ull) & (tj > ti));
+}
+__device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ uint64_t query_key(float rho, uint32_t tag) {
+  return rho_tag_key(rho, tag, true);
+}
+__device__ __forceinline__ uint64_t query_key(double rho, uint32_t) {
+  return rho_key_f64(rho, true);
+}
+
+template <int D, int M, class T>
 __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
-    const RunParams rp, const EventDesc* __restrict__ events, const float* __restrict__ geo,
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
-    const int32_t* __restrict__ n_items, const float* __restrict__ sx, int64_t cs,
-    const float* __restrict__ ssig, const float* __restrict__ su,
-    const uint32_t* __restrict__ stag, const float* __restrict__ srho,
+    const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
+    const T* __restrict__ ssig, const T* __restrict__ su,
+    const uint32_t* __restrict__ stag, const T* __restrict__ srho,
     const uint64_t* __restrict__ skey, const int32_t* __restrict__ qperm,
-    int32_t* __restrict__ snh, float* __restrict__ sdelta) {
+    int32_t* __restrict__ snh, T* __restrict__ sdelta) {
+  constexpr bool kF64 = sizeof(T) == 8;
   constexpr int U = Unroll<D>::U_NH;
   const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
                    threadIdx.x / kWave;
   if (item >= *n_items) return;
   const Item it = items[item];
   const EventDesc ev = events[it.event];
-  const Geo g = event_geo(geo, rp, it.event, ev.nper);
+  const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
   const int q = it.begin + lane_id();
   const bool active = q < it.end;
   const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
-  float xi[D], si[D];
+  T xi[D], si[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    xi[d] = active ? sx[d * cs + s] : 0.0f;
-    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : 0.0f;
+    xi[d] = active ? sx[d * cs + s] : T(0);
+    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
   }
-  const float rho_i = active ? srho[s] : 0.0f;
-  const uint64_t key_i = active ? rho_tag_key(rho_i, stag[s], true) : ~0ull;
-  const float ui = (active && rp.has_u) ? su[s] : 1.0f;
-  const float eff_min = rp.rhoc * ui;
+  const T rho_i = active ? srho[s] : T(0);
+  const uint32_t tag_i = active ? stag[s] : 0u;
+  const uint64_t key_i = active ? query_key(rho_i, tag_i) : ~0ull;
+  const T ui = (active && rp.has_u) ? su[s] : T(1);
+  const T eff_min = rp.rhoc * ui;
   const bool use_seed = rho_i >= eff_min;
-  const float eff = use_seed ? rp.seed_dc : rp.dm;
-  const float eff_thr =
+  const T eff = use_seed ? rp.seed_dc : rp.dm;
+  const T eff_thr =
       MetricTraits<M>::kSqrt ? (use_seed ? rp.seed_sq_max : rp.dm_sq_max) : eff;
 
   int lo[D], hi[D];
@@ -741,15 +810,15 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
       hi[d] = INT32_MIN;
     }
   }
-  float delta = FLT_MAX;
+  T delta = std::numeric_limits<T>::max();
   int nh = -1;
-  float rho_nh = 0.0f;
+  T rho_nh = T(0);
   uint32_t tag_nh = 0u;
-  float bound = eff_thr;  // prune_threshold(eff_thr, delta)
+  T bound = eff_thr;  // prune_threshold(eff_thr, delta)
 
   // The reference's acceptance test, in candidate order.
-  auto consider = [&](float v, float rj, uint32_t tj, int j) {
-    const float dist = finish_distance<M>(v);
+  auto consider = [&](T v, T rj, uint32_t tj, int j) {
+    const T dist = finish_distance<M>(v);
     if (dist <= eff &&
         ((dist < delta) ||
          ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
@@ -769,30 +838,35 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
     for (; j + U <= j1; j += U) {
       // m: the smallest distance among the block's higher candidates (vector
       // selects and minima instead of per-candidate scalar mask logic).
-      float v[U];
-      float m = __builtin_inff();
+      T v[U];
+      uint32_t tj[U];
+      T m = std::numeric_limits<T>::infinity();
+#pragma unroll
+      for (int k = 0; k < U; ++k) tj[k] = kF64 ? stag[j + k] : 0u;
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
-        m = __builtin_fminf(m, (skey[j + k] > key_i) ? v[k] : __builtin_inff());
+        const bool h = key_higher(skey[j + k], tj[k], key_i, tag_i, T(0));
+        m = fmin_t(m, h ? v[k] : std::numeric_limits<T>::infinity());
       }
       // Most blocks hold no candidate for any lane: one wave-wide test skips them.
       if (ballot(m <= bound) != 0ull) {
-        float rj[U];
-        uint32_t tj[U];
+        T rj[U];
 #pragma unroll
         for (int k = 0; k < U; ++k) {
           rj[k] = srho[j + k];
-          tj[k] = stag[j + k];
+          if (!kF64) tj[k] = stag[j + k];
         }
 #pragma unroll
         for (int k = 0; k < U; ++k)
-          if ((skey[j + k] > key_i) && v[k] <= bound) consider(v[k], rj[k], tj[k], j + k);
+          if (key_higher(skey[j + k], tj[k], key_i, tag_i, T(0)) && v[k] <= bound)
+            consider(v[k], rj[k], tj[k], j + k);
       }
     }
     for (; j < j1; ++j) {
-      const float v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
-      if ((skey[j] > key_i) & (v <= bound)) consider(v, srho[j], stag[j], j);
+      const T v = metric_value<D, M>(xi, si, sx, cs, j, ssig, rp);
+      if (key_higher(skey[j], stag[j], key_i, tag_i, T(0)) & (v <= bound))
+        consider(v, srho[j], stag[j], j);
     }
   });
   if (active) {
@@ -807,17 +881,18 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
 // -2 for a seed, -1 for a point without nearest-higher, else the sorted
 // position of its nearest-higher.
 // ---------------------------------------------------------------------------
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_unpermute(
-    const RunParams rp, int n, const int32_t* __restrict__ perm, const float* __restrict__ srho,
-    int32_t* __restrict__ snh, const float* __restrict__ sdelta, const float* __restrict__ su,
-    float* __restrict__ rho, int32_t* __restrict__ nh, float* __restrict__ delta,
+    const RunParamsT<T> rp, int n, const int32_t* __restrict__ perm, const T* __restrict__ srho,
+    int32_t* __restrict__ snh, const T* __restrict__ sdelta, const T* __restrict__ su,
+    T* __restrict__ rho, int32_t* __restrict__ nh, T* __restrict__ delta,
     int32_t* __restrict__ is_seed, int32_t* __restrict__ flag) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int i = perm[s];
-  const float r = srho[s];
+  const T r = srho[s];
   const int h = snh[s];
-  const float u = rp.has_u ? su[s] : 1.0f;
+  const T u = rp.has_u ? su[s] : T(1);
   const int seed = (h == -1) && (r >= rp.rhoc * u) ? 1 : 0;
   rho[i] = r;
   nh[i] = h >= 0 ? perm[h] : -1;
@@ -866,25 +941,26 @@ __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chu
 // ---------------------------------------------------------------------------
 // Dispatch over (D, metric, kernel).
 // ---------------------------------------------------------------------------
+template <class T>
 struct NeighbourLaunch {
-  const RunParams* rp;
-  const BatchBuffers* bb;
+  const RunParamsT<T>* rp;
+  const BatchBuffersT<T>* bb;
   int64_t cs;
   unsigned grid;
   unsigned block;  // threads per workgroup of the neighbour stages
   hipStream_t s;
 };
 
-template <int D, int M, int K>
-static void launch_density_dmk(const NeighbourLaunch& L) {
-  const BatchBuffers& b = *L.bb;
-  hipLaunchKernelGGL((k_density<D, M, K>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
+template <int D, int M, int K, class T>
+static void launch_density_dmk(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
+  hipLaunchKernelGGL((k_density<D, M, K, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
                      b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
                      b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
 }
 
-template <int D, int M>
-static void launch_density_dm(const NeighbourLaunch& L) {
+template <int D, int M, class T>
+static void launch_density_dm(const NeighbourLaunch<T>& L) {
   switch (L.rp->kernel) {
     case K_FLAT: launch_density_dmk<D, M, K_FLAT>(L); break;
     case K_EXP: launch_density_dmk<D, M, K_EXP>(L); break;
@@ -897,17 +973,17 @@ static void launch_density_dm(const NeighbourLaunch& L) {
   }
 }
 
-template <int D, int M>
-static void launch_nh_dm(const NeighbourLaunch& L) {
-  const BatchBuffers& b = *L.bb;
-  hipLaunchKernelGGL((k_nearest_higher<D, M>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp,
+template <int D, int M, class T>
+static void launch_nh_dm(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
+  hipLaunchKernelGGL((k_nearest_higher<D, M, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp,
                      b.events, b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs,
                      b.sorted.sigma[0], b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm_nh,
                      b.snh, b.sdelta);
 }
 
-template <int D>
-static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
+template <int D, class T>
+static void launch_neighbour_d(const NeighbourLaunch<T>& L, bool density) {
 #define CLUE_METRIC_CASE(MID)               \
   case MID:                                 \
     if (density)                            \
@@ -931,17 +1007,18 @@ static void launch_neighbour_d(const NeighbourLaunch& L, bool density) {
 #undef CLUE_METRIC_CASE
 }
 
-template <int D>
-static void launch_query_order_d(const NeighbourLaunch& L) {
-  const BatchBuffers& b = *L.bb;
+template <int D, class T>
+static void launch_query_order_d(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
   const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((k_query_order<D>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
+  hipLaunchKernelGGL((k_query_order<D, T>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
                      b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
                      b.sorted.sigma[0], b.qkey, b.qfine, b.qperm, b.qperm_nh);
 }
 
 // The query orders of both neighbour stages (qperm for dc, qperm_nh for dm).
-static void launch_query_order(const NeighbourLaunch& L) {
+template <class T>
+static void launch_query_order(const NeighbourLaunch<T>& L) {
   switch (L.rp->D) {
     case 1: launch_query_order_d<1>(L); break;
     case 2: launch_query_order_d<2>(L); break;
@@ -956,7 +1033,8 @@ static void launch_query_order(const NeighbourLaunch& L) {
   }
 }
 
-static void launch_neighbour(const NeighbourLaunch& L, bool density) {
+template <class T>
+static void launch_neighbour(const NeighbourLaunch<T>& L, bool density) {
   switch (L.rp->D) {
     case 1: launch_neighbour_d<1>(L, density); break;
     case 2: launch_neighbour_d<2>(L, density); break;
@@ -977,37 +1055,39 @@ const char* stage_name(int s) {
   return (s >= 0 && s < kNumStages) ? names[s] : "?";
 }
 
-void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuffers& bb,
-                     const ResultArrays& out, hipStream_t s, const StageHook* hook) {
+template <class T>
+void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const BatchBuffersT<T>& bb,
+                     const ResultArraysT<T>& out, hipStream_t s, const StageHook* hook) {
+  using I = typename FloatBits<T>::Int;
   auto mark = [&](int stage) {
     if (hook && hook->mark) hook->mark(hook->self, stage, s);
   };
   const int D = rp.D;
   const int n = bb.n_total;
-  CoordPtrs cp{};
+  CoordPtrs<T> cp{};
   for (int d = 0; d < D; ++d) cp.x[d] = in.x[d];
 
   // (a) tiles
   mark(kStageExtremes);
-  hipLaunchKernelGGL(k_init_extremes,
+  hipLaunchKernelGGL(k_init_extremes<I>,
                      dim3(static_cast<unsigned>((static_cast<int64_t>(bb.n_events) * D + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, s, bb.ext, static_cast<int64_t>(bb.n_events) * D);
-  hipLaunchKernelGGL(k_extremes, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, D, bb.chunks, bb.ext);
-  hipLaunchKernelGGL(k_geometry, dim3((bb.n_events + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(k_extremes<T>, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, D, bb.chunks, bb.ext);
+  hipLaunchKernelGGL(k_geometry<T>, dim3((bb.n_events + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
                      bb.ext, bb.events, bb.n_events, D, bb.geo);
   mark(kStageTiles);
-  hipLaunchKernelGGL(k_tile_keys, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, rp, bb.chunks,
+  hipLaunchKernelGGL(k_tile_keys<T>, dim3(bb.n_chunks), dim3(kBlock), 0, s, cp, rp, bb.chunks,
                      bb.events, bb.geo, bb.keys0, bb.vals0);
   bool in_first = true;
   radix_sort_pairs(bb.keys0, bb.vals0, bb.keys1, bb.vals1, n, bb.key_bits, bb.rs_hist, bb.scan_tmp,
                    &in_first, s);
   const uint32_t* skeys = in_first ? bb.keys0 : bb.keys1;
   const int32_t* sperm = in_first ? bb.vals0 : bb.vals1;
-  GatherArgs ga;
+  GatherArgs<T> ga;
   ga.in = in;
   ga.out = bb.sorted;
   ga.cs = 0;
-  hipLaunchKernelGGL(k_gather, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ga, rp, skeys,
+  hipLaunchKernelGGL(k_gather<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ga, rp, skeys,
                      sperm, n, bb.n_tiles_total, bb.toff);
   hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      s, bb.toff, bb.n_tiles_total, bb.icount);
@@ -1015,7 +1095,7 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
   hipLaunchKernelGGL(k_item_fill, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items);
 
-  NeighbourLaunch L;
+  NeighbourLaunch<T> L;
   L.rp = &rp;
   L.bb = &bb;
   L.cs = bb.sorted_stride;
@@ -1034,7 +1114,7 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
 
   // (d), (e)
   mark(kStageSeeds);
-  hipLaunchKernelGGL(k_unpermute, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
+  hipLaunchKernelGGL(k_unpermute<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
                      bb.sorted.perm, bb.srho, bb.snh, bb.sdelta, bb.sorted.u, out.rho, out.nh,
                      out.delta, out.is_seed, bb.flag);
   scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
@@ -1044,5 +1124,12 @@ void launch_pipeline(const RunParams& rp, const PointArrays& in, const BatchBuff
                      out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
+
+template void launch_pipeline<float>(const RunParamsT<float>&, const PointArraysT<float>&,
+                                     const BatchBuffersT<float>&, const ResultArraysT<float>&,
+                                     hipStream_t, const StageHook*);
+template void launch_pipeline<double>(const RunParamsT<double>&, const PointArraysT<double>&,
+                                      const BatchBuffersT<double>&, const ResultArraysT<double>&,
+                                      hipStream_t, const StageHook*);
 
 }  // namespace clue_dev

@@ -35,8 +35,10 @@ struct Item {
   int32_t event;
 };
 
-// Per-run scalar parameters as the device stages consume them.
-struct RunParams {
+// Per-run scalar parameters as the device stages consume them; T is the point
+// type (float, or double for Clusterer<Ndim, double>).
+template <class T>
+struct RunParamsT {
   int32_t D;
   int32_t kernel;
   int32_t metric;
@@ -45,22 +47,23 @@ struct RunParams {
   int32_t has_tags;
   int32_t n_events;
   int32_t n_total;
-  float dc;
-  float rhoc;
-  float dm;
-  float seed_dc;
-  float dc_sq_max;      // largest float t with sqrt_rn(t) <= dc (sqrt metrics)
-  float dm_sq_max;      // same for dm
-  float seed_sq_max;    // same for seed_dc
-  float kp0, kp1;       // kernel parameters (flat f | exp lambda, A | gauss sigma, A)
-  float gauss_den;      // (2 * sigma) * sigma, as the reference evaluates it
-  float gauss_inv_den;  // 1 / gauss_den when gauss_den is a power of two
-  int32_t gauss_pow2;   // gauss_den is a power of two
-  int32_t exp_in_range; // every kernel exp argument within the radius lies in [-103, 0]
+  T dc;
+  T rhoc;
+  T dm;
+  T seed_dc;
+  T dc_sq_max;           // largest t with sqrt_rn(t) <= dc (sqrt metrics)
+  T dm_sq_max;           // same for dm
+  T seed_sq_max;         // same for seed_dc
+  T kp0, kp1;            // kernel parameters (flat f | exp lambda, A | gauss sigma, A)
+  T gauss_den;           // (2 * sigma) * sigma, as the reference evaluates it
+  T gauss_inv_den;       // 1 / gauss_den when gauss_den is a power of two
+  int32_t gauss_pow2;    // gauss_den is a power of two
+  int32_t exp_in_range;  // every kernel exp argument within the radius is in the core range
   int32_t wrapped[kMaxDim];
-  float mparam[kMaxDim];
+  T mparam[kMaxDim];
   int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none
   int32_t block_waves;     // neighbour stages: waves per workgroup (1, 2, 4, 8 or 16)
 };
+using RunParams = RunParamsT<float>;
 
 }  // namespace clue_dev

@@ -151,21 +151,24 @@ class PointsDevice:
     weights, and rho / nearest_higher / is_seed / cluster_index outputs, plus the
     optional per-dimension sigma, density uncertainty and tags."""
 
-    def __init__(self, ctx: Context, n_points: int, ndim: int):
+    def __init__(self, ctx: Context, n_points: int, ndim: int, dtype=np.float32):
         if n_points < 1:
             raise ValueError("the number of points must be positive")
         if not 1 <= ndim <= _capi.MAX_DIM:
             raise IndexError("This library only works up to 10 dimensions")
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise TypeError("points are float32 or float64")
         self.ctx = ctx
         self.n = int(n_points)
         self.ndim = int(ndim)
-        self.coords = ctx.empty((ndim, n_points), np.float32)
-        self.weight = ctx.empty(n_points, np.float32)
+        self.coords = ctx.empty((ndim, n_points), self.dtype)
+        self.weight = ctx.empty(n_points, self.dtype)
         self.cluster_index = ctx.empty(n_points, np.int32)
         self.is_seed = ctx.empty(n_points, np.int32)
-        self.rho = ctx.empty(n_points, np.float32)
+        self.rho = ctx.empty(n_points, self.dtype)
         self.nearest_higher = ctx.empty(n_points, np.int32)
-        self.delta = ctx.empty(n_points, np.float32)
+        self.delta = ctx.empty(n_points, self.dtype)
         self.sigma: Optional[DeviceArray] = None
         self.density_uncertainty: Optional[DeviceArray] = None
         self.tags: Optional[DeviceArray] = None
@@ -173,13 +176,13 @@ class PointsDevice:
 
     @classmethod
     def from_host(cls, ctx: Context, coords: np.ndarray, weight: np.ndarray, sigma=None, u=None,
-                  tags=None) -> "PointsDevice":
-        c = np.ascontiguousarray(coords, dtype=np.float32)
+                  tags=None, dtype=np.float32) -> "PointsDevice":
+        c = np.ascontiguousarray(coords, dtype=dtype)
         if c.ndim == 1:
             c = c.reshape(1, -1)
-        p = cls(ctx, c.shape[1], c.shape[0])
+        p = cls(ctx, c.shape[1], c.shape[0], dtype)
         p.coords.copy_from_host(c)
-        p.weight.copy_from_host(np.ascontiguousarray(weight, dtype=np.float32))
+        p.weight.copy_from_host(np.ascontiguousarray(weight, dtype=dtype))
         if sigma is not None:
             p.set_sigmas(sigma)
         if u is not None:
@@ -189,17 +192,21 @@ class PointsDevice:
         return p
 
     def set_sigmas(self, sigma: np.ndarray) -> None:
-        s = np.ascontiguousarray(sigma, dtype=np.float32).reshape(self.ndim, self.n)
+        s = np.ascontiguousarray(sigma, dtype=self.dtype).reshape(self.ndim, self.n)
         self.sigma = self.ctx.to_device(s)
 
     def set_density_uncertainty(self, u: np.ndarray) -> None:
-        self.density_uncertainty = self.ctx.to_device(np.asarray(u, np.float32).reshape(self.n))
+        self.density_uncertainty = self.ctx.to_device(np.asarray(u, self.dtype).reshape(self.n))
 
     def set_tags(self, tags: np.ndarray) -> None:
         self.tags = self.ctx.to_device(np.asarray(tags, np.uint32).reshape(self.n))
 
-    def c_points(self) -> _capi.Points:
-        pts = _capi.Points()
+    @property
+    def f64(self) -> bool:
+        return self.dtype == np.float64
+
+    def c_points(self):
+        pts = _capi.PointsF64() if self.f64 else _capi.Points()
         pts.n = self.n
         for d in range(self.ndim):
             pts.coords[d] = self.coords.address(d * self.n)
@@ -210,8 +217,8 @@ class PointsDevice:
         pts.tags = self.tags.ptr if self.tags else None
         return pts
 
-    def c_results(self) -> _capi.Results:
-        r = _capi.Results()
+    def c_results(self):
+        r = _capi.ResultsF64() if self.f64 else _capi.Results()
         r.cluster_index = self.cluster_index.ptr
         r.is_seed = self.is_seed.ptr
         r.rho = self.rho.ptr
@@ -268,7 +275,7 @@ class Clusterer:
         self.wrapped = tuple(int(w) for w in wrapped)
 
     def params(self, ndim: int, metric: Optional[MetricDescriptor] = None,
-               kernel=None) -> _capi.Params:
+               kernel=None, dtype=np.float32):
         metric = metric if metric is not None else MetricDescriptor("euclidean_plain")
         kernel = kernel if kernel is not None else FlatKernel(0.5)
         if metric.tag == "euclidean_plain":
@@ -277,18 +284,23 @@ class Clusterer:
             mid, mp = metric.resolve(ndim)
         kid, kp = _kernel_params(kernel)
         p = _capi.make_params(ndim, self.dc, self.rhoc, self.dm, self.seed_dc, kid, kp, mid, mp,
-                              self.wrapped, self.points_per_tile)
-        _capi.check(_capi.load().clue_validate_params(C.byref(p)))
+                              self.wrapped, self.points_per_tile, dtype)
+        lib = _capi.load()
+        if np.dtype(dtype) == np.float64:
+            _capi.check(lib.clue_validate_params_f64(C.byref(p)))
+        else:
+            _capi.check(lib.clue_validate_params(C.byref(p)))
         return p
 
     def make_clusters(self, points: PointsDevice, metric: Optional[MetricDescriptor] = None,
                       kernel=None) -> None:
         """make_clusters(queue, d_points, metric, kernel) (core/detail/Clusterer.hpp:143-151)."""
-        p = self.params(points.ndim, metric, kernel)
+        p = self.params(points.ndim, metric, kernel, points.dtype)
         pts = points.c_points()
         res = points.c_results()
-        _capi.check(_capi.load().clue_make_clusters(self.ctx.handle, C.byref(p), C.byref(pts),
-                                                    C.byref(res)))
+        lib = _capi.load()
+        fn = lib.clue_make_clusters_f64 if points.f64 else lib.clue_make_clusters
+        _capi.check(fn(self.ctx.handle, C.byref(p), C.byref(pts), C.byref(res)))
         points.clustered = True
 
     def make_clusters_batched(self, points: PointsDevice, sizes: Sequence[int],
@@ -296,12 +308,14 @@ class Clusterer:
                               event_of_cluster: Optional[DeviceArray] = None) -> int:
         """make_clusters(queue, d_points, batch_item_sizes, ...) (core/detail/Clusterer.hpp:171-183).
         Returns the number of clusters."""
-        p = self.params(points.ndim, metric, kernel)
+        p = self.params(points.ndim, metric, kernel, points.dtype)
         sz = _capi.as_u32_array(sizes)
         pts = points.c_points()
         res = points.c_results()
         nc = C.c_int64()
-        _capi.check(_capi.load().clue_make_clusters_batched(
+        lib = _capi.load()
+        fn = lib.clue_make_clusters_batched_f64 if points.f64 else lib.clue_make_clusters_batched
+        _capi.check(fn(
             self.ctx.handle, C.byref(p), C.byref(pts), sz.ctypes.data_as(C.POINTER(C.c_uint32)),
             len(sz), C.byref(res), C.c_void_p(event_of_cluster.ptr if event_of_cluster else None),
             C.byref(nc)))
@@ -405,6 +419,18 @@ def device_count() -> int:
     c = C.c_int32()
     st = lib.clue_device_count(C.byref(c))
     return c.value if st == 0 else 0
+
+
+def debug_math_f64(ctx: Context, a: np.ndarray, b: np.ndarray):
+    """Device sqrt(a), exp(a) and a / b in double precision, as the stages compute them."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    da, db = ctx.to_device(a), ctx.to_device(b)
+    o1, o2, o3 = (ctx.empty(a.shape, np.float64) for _ in range(3))
+    _capi.check(_capi.load().clue_debug_math_f64(ctx.handle, a.size, C.c_void_p(da.ptr),
+                                                 C.c_void_p(db.ptr), C.c_void_p(o1.ptr),
+                                                 C.c_void_p(o2.ptr), C.c_void_p(o3.ptr)))
+    return o1.to_host(), o2.to_host(), o3.to_host()
 
 
 def debug_math(ctx: Context, a: np.ndarray, b: np.ndarray):

@@ -143,6 +143,58 @@ clue_status clue_make_clusters_host(clue_ctx* ctx, const clue_params* params, in
                                     int32_t* host_is_seed, float* host_rho,
                                     int32_t* host_nearest_higher);
 
+/* Double precision: Clusterer<Ndim, double> with PointsHost / PointsDevice<Ndim, double>
+ * (core/Clusterer.hpp:37-51, value_type = double). Same fields and meaning as the
+ * single-precision structs above, with double parameters, coordinates, weights and
+ * outputs; every stage computes in double. Graph plans and the server are
+ * single precision only. */
+typedef struct {
+  int32_t ndim;
+  int32_t kernel;
+  int32_t metric;
+  int32_t points_per_tile;
+  int32_t wrapped[10];
+  double dc;
+  double rhoc;
+  double dm;
+  double seed_dc;
+  double kparam[2];
+  double mparam[10];
+} clue_params_f64;
+
+typedef struct {
+  int32_t n;
+  const double* coords[10];
+  const double* weight;
+  const double* sigma[10];
+  const double* density_uncertainty;
+  const uint32_t* tags;
+} clue_points_f64;
+
+typedef struct {
+  int32_t* cluster_index;
+  int32_t* is_seed;
+  double* rho;
+  int32_t* nearest_higher;
+  double* delta;
+} clue_results_f64;
+
+clue_status clue_validate_params_f64(const clue_params_f64* params);
+/* clue_make_clusters for double points (core/detail/Clusterer.hpp:143-151). Waits. */
+clue_status clue_make_clusters_f64(clue_ctx* ctx, const clue_params_f64* params,
+                                   const clue_points_f64* points, clue_results_f64* results);
+/* clue_make_clusters_batched for double points (core/detail/Clusterer.hpp:171-183). */
+clue_status clue_make_clusters_batched_f64(clue_ctx* ctx, const clue_params_f64* params,
+                                           const clue_points_f64* points, const uint32_t* sizes,
+                                           int32_t n_events, clue_results_f64* results,
+                                           int32_t* event_of_cluster, int64_t* n_clusters);
+/* clue_make_clusters_host for a (ndim+1) x n double SoA (core/detail/Clusterer.hpp:127-138). */
+clue_status clue_make_clusters_host_f64(clue_ctx* ctx, const clue_params_f64* params, int32_t n,
+                                        const double* host_data, int32_t* host_cluster_index,
+                                        const uint32_t* sizes, int32_t n_events,
+                                        int32_t* host_is_seed, double* host_rho,
+                                        int32_t* host_nearest_higher);
+
 /* Seeds of the last run, ascending point index (getSeeds, core/detail/Clusterer.hpp:198-203).
  * `seeds` is a HOST array of capacity `capacity`; *count receives the number of seeds.
  * CLUE_ERR_RUNTIME before the first run. */
@@ -259,6 +311,9 @@ clue_status clue_debug_math(clue_ctx* ctx, int32_t n, const float* a, const floa
 /* A plain device copy of `bytes` (multiple of 16), the calibration point of the
  * HBM traffic counters (FETCH_SIZE / WRITE_SIZE). No reference counterpart. */
 clue_status clue_debug_copy(clue_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* The double-precision stage math (sqrt, exp, division) on device arrays. */
+clue_status clue_debug_math_f64(clue_ctx* ctx, int32_t n, const double* a, const double* b,
+                                double* out_sqrt, double* out_exp, double* out_div);
 /* Test hook: fills every workspace buffer of the context with 0xFF bytes (NaN
  * floats, -1 integers), so that a later run cannot pass on values left by an
  * earlier one. Waits. */

@@ -46,6 +46,36 @@ float expf_cw(float x) {
   return std::ldexp(p, static_cast<int>(k));
 }
 
+// Double precision: the same method with a 29-bit ln2_hi (k*ln2_hi is exact for
+// |k| <= 1075) and the degree-13 Taylor polynomial (remainder below 2^-57 on
+// |r| <= ln2/2). Coefficients are 1/n! rounded to double.
+double expd_cw(double x) {
+  if (x != x) return x;
+  if (x > 0x1.62e42fefa39efp+9) return INFINITY;  // > ln(DBL_MAX)
+  if (x < -0x1.74910d52d3052p+9) return 0.0;      // e^x < 2^-1075 rounds to +0
+  const double log2e = 0x1.71547652b82fep+0;
+  const double ln2_hi = 0x1.62e42ff000000p-1;
+  const double ln2_lo = -0x1.718432a1b0e26p-35;
+  const double k = std::rint(x * log2e);
+  double r = std::fma(-k, ln2_hi, x);
+  r = std::fma(-k, ln2_lo, r);
+  double p = 0x1.6124613a86d09p-33;       // 1/13!
+  p = std::fma(p, r, 0x1.1eed8eff8d898p-29);  // 1/12!
+  p = std::fma(p, r, 0x1.ae64567f544e4p-26);  // 1/11!
+  p = std::fma(p, r, 0x1.27e4fb7789f5cp-22);  // 1/10!
+  p = std::fma(p, r, 0x1.71de3a556c734p-19);  // 1/9!
+  p = std::fma(p, r, 0x1.a01a01a01a01ap-16);  // 1/8!
+  p = std::fma(p, r, 0x1.a01a01a01a01ap-13);  // 1/7!
+  p = std::fma(p, r, 0x1.6c16c16c16c17p-10);  // 1/6!
+  p = std::fma(p, r, 0x1.1111111111111p-7);   // 1/5!
+  p = std::fma(p, r, 0x1.5555555555555p-5);   // 1/4!
+  p = std::fma(p, r, 0x1.5555555555555p-3);   // 1/3!
+  p = std::fma(p, r, 0x1.0000000000000p-1);   // 1/2!
+  p = std::fma(p, r, 1.0);
+  p = std::fma(p, r, 1.0);
+  return std::ldexp(p, static_cast<int>(k));
+}
+
 template <class T>
 T exp_t(T x);
 template <>
@@ -54,7 +84,7 @@ float exp_t<float>(float x) {
 }
 template <>
 double exp_t<double>(double x) {
-  return std::exp(x);
+  return expd_cw(x);
 }
 
 // static_cast<int>(float) as the x86 reference executes it (cvttss2si): NaN and
@@ -458,6 +488,10 @@ int run_single(const oracle_params* p, int32_t n, const T* coords, const T* weig
 extern "C" {
 
 float oracle_expf(float x) { return expf_cw(x); }
+
+double oracle_exp_f64(double x) { return expd_cw(x); }
+
+double oracle_expl_f64(double x) { return static_cast<double>(std::exp(static_cast<long double>(x))); }
 
 int oracle_num_threads(void) {
 #ifdef _OPENMP

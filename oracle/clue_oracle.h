@@ -132,6 +132,10 @@ int oracle_pair_counts_f32(const oracle_params* p, int32_t n, const float* coord
                            int64_t* in_radius, int64_t* visited_nh, int64_t* nh_pairs);
 
 float oracle_expf(float x);
+/* The double-precision exp restatement, and exp in long double rounded to double
+ * (the accuracy reference of the tests). */
+double oracle_exp_f64(double x);
+double oracle_expl_f64(double x);
 int oracle_num_threads(void);
 
 #ifdef __cplusplus

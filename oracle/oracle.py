@@ -100,6 +100,9 @@ def lib(omp: bool = False) -> C.CDLL:
     h = C.CDLL(path)
     h.oracle_expf.restype = C.c_float
     h.oracle_expf.argtypes = [C.c_float]
+    for fn in (h.oracle_exp_f64, h.oracle_expl_f64):
+        fn.restype = C.c_double
+        fn.argtypes = [C.c_double]
     _libs[name] = h
     return h
 
@@ -313,6 +316,18 @@ def expf(x: float) -> float:
 def expf_array(xs: np.ndarray) -> np.ndarray:
     h = lib()
     return np.array([h.oracle_expf(C.c_float(float(x))) for x in xs], dtype=np.float32)
+
+
+def exp_f64_array(xs: np.ndarray) -> np.ndarray:
+    """The double-precision exp restatement of the oracle (and of the device)."""
+    h = lib()
+    return np.array([h.oracle_exp_f64(float(x)) for x in xs], dtype=np.float64)
+
+
+def expl_f64_array(xs: np.ndarray) -> np.ndarray:
+    """exp evaluated in long double and rounded to double: the accuracy reference."""
+    h = lib()
+    return np.array([h.oracle_expl_f64(float(x)) for x in xs], dtype=np.float64)
 
 
 def num_threads(omp: bool = True) -> int:

@@ -24,18 +24,21 @@ class Case:
 
     def __init__(self, coords, weight, dc, rhoc, dm=None, seed_dc=None, kernel=None,
                  metric=None, wrapped=(), sigma=None, u=None, tags=None, sizes=None,
-                 points_per_tile=0, name="case"):
-        self.coords = np.ascontiguousarray(coords, np.float32)
+                 points_per_tile=0, name="case", dtype=np.float32):
+        self.dtype = np.dtype(dtype)
+        self.coords = np.ascontiguousarray(coords, self.dtype)
         if self.coords.ndim == 1:
             self.coords = self.coords.reshape(1, -1)
-        self.weight = np.ascontiguousarray(weight, np.float32)
+        self.weight = np.ascontiguousarray(weight, self.dtype)
         self.dc, self.rhoc = float(dc), float(rhoc)
         self.dm = float(dc if dm is None else dm)
         self.seed_dc = float(dc if seed_dc is None else seed_dc)
         self.kernel = kernel if kernel is not None else K.FlatKernel(0.5)
         self.metric = metric  # None: plain Euclidean
         self.wrapped = tuple(wrapped)
-        self.sigma, self.u, self.tags = sigma, u, tags
+        self.sigma = None if sigma is None else np.ascontiguousarray(sigma, self.dtype)
+        self.u = None if u is None else np.ascontiguousarray(u, self.dtype)
+        self.tags = tags
         self.sizes = sizes
         self.points_per_tile = points_per_tile
         self.name = name
@@ -58,16 +61,38 @@ class Case:
         return oracle.Config(ndim=self.ndim, dc=self.dc, rhoc=self.rhoc, dm=self.dm,
                              seed_dc=self.seed_dc, kernel=KERNEL_NAMES[self.kernel.kind],
                              kparams=self.kernel.params, metric=METRIC_NAMES[mid],
-                             mparams=[float(np.float32(x)) for x in mp], wrapped=self.wrapped,
+                             mparams=[float(self.dtype.type(x)) for x in mp], wrapped=self.wrapped,
                              points_per_tile=self.points_per_tile or 128)
 
     def run_oracle(self, oracle, omp=True):
         cfg = self.oracle_config(oracle)
         if self.sizes is None:
             return oracle.run(cfg, self.coords, self.weight, sigma=self.sigma, u=self.u,
-                              tags=self.tags, omp=omp)
-        return oracle.run_batched(cfg, self.coords, self.weight, self.sizes, sigma=self.sigma,
-                                  u=self.u, tags=self.tags, omp=omp)
+                              tags=self.tags, omp=omp, dtype=self.dtype)
+        if self.dtype == np.float32:
+            return oracle.run_batched(cfg, self.coords, self.weight, self.sizes, sigma=self.sigma,
+                                      u=self.u, tags=self.tags, omp=omp)
+        return self._run_oracle_per_event(oracle, cfg, omp)
+
+    def _run_oracle_per_event(self, oracle, cfg, omp):
+        """Batched semantics from single-event oracle runs: each event alone, ids
+        global and event-major, nearest-higher as packed indices, default tags the
+        packed index (BatchedClusteringKernels.hpp:183-187, 291-336)."""
+        outs, first, off = [], 0, 0
+        for n in (int(x) for x in self.sizes):
+            sl = slice(off, off + n)
+            tags = self.tags[sl] if self.tags is not None else np.arange(off, off + n, dtype=np.uint32)
+            r = oracle.run(cfg, self.coords[:, sl], self.weight[sl],
+                           sigma=None if self.sigma is None else self.sigma[:, sl],
+                           u=None if self.u is None else self.u[sl], tags=tags, omp=omp,
+                           dtype=self.dtype)
+            nh = np.where(r.nh >= 0, r.nh + off, -1).astype(np.int32)
+            cl = np.where(r.cluster >= 0, r.cluster + first, -1).astype(np.int32)
+            outs.append((r.rho, nh, r.delta, r.is_seed, cl))
+            first += r.n_seeds
+            off += n
+        cat = [np.concatenate([o[k] for o in outs]) for k in range(5)]
+        return oracle.Result(cat[0], cat[1], cat[2], cat[3], cat[4], first, None, None)
 
     def clusterer(self, ctx):
         from cluestering_amd.device import Clusterer
@@ -79,7 +104,7 @@ class Case:
     def points(self, ctx):
         from cluestering_amd.device import PointsDevice
         return PointsDevice.from_host(ctx, self.coords, self.weight, sigma=self.sigma, u=self.u,
-                                      tags=self.tags)
+                                      tags=self.tags, dtype=self.dtype)
 
     def run_gpu(self, ctx):
         c = self.clusterer(ctx)
@@ -99,7 +124,9 @@ def assert_same(gpu: dict, ref, name: str = "") -> None:
     """Bit-exact comparison of every output (rho and delta by their bits)."""
     where = f" [{name}]" if name else ""
     for key, a, b in (("rho", gpu["rho"], ref.rho), ("delta", gpu["delta"], ref.delta)):
-        bad = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
+        assert a.dtype == b.dtype, f"{key}: gpu {a.dtype} oracle {b.dtype}{where}"
+        bits = np.uint32 if a.dtype == np.float32 else np.uint64
+        bad = np.nonzero(a.view(bits) != b.view(bits))[0]
         assert bad.size == 0, (f"{key} differs at {bad.size} points{where}, first {bad[:5]}: "
                                f"gpu {a[bad[:5]]} oracle {b[bad[:5]]}")
     for key, a, b in (("nearest_higher", gpu["nearest_higher"], ref.nh),

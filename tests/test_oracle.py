@@ -28,6 +28,46 @@ def test_expf_within_one_ulp(oracle_mod):
     assert math.isinf(oracle_mod.expf(100.0))
 
 
+def test_exp_f64_within_one_ulp(oracle_mod):
+    """The double exp restatement (29-bit ln2_hi Cody-Waite, degree-13 Taylor) is
+    within 1 ulp of exp evaluated in long double and rounded, over the whole
+    finite range, the denormal results included; and exact at the special points."""
+    rng = np.random.default_rng(9)
+    x = np.concatenate([rng.uniform(-745.2, 709.8, 60000), rng.uniform(-1, 1, 20000),
+                        rng.uniform(-745.2, -708.0, 10000), rng.uniform(700, 709.78, 5000)])
+    a = oracle_mod.exp_f64_array(x)
+    b = oracle_mod.expl_f64_array(x)
+    ulp = np.abs(a.view(np.int64) - b.view(np.int64))
+    assert ulp.max() <= 1
+    sp = np.array([0.0, -0.0, 709.8, -745.2, np.inf, -np.inf, np.nan])
+    e = oracle_mod.exp_f64_array(sp)
+    assert e[0] == 1.0 and e[1] == 1.0 and e[2] == np.inf and e[3] == 0.0
+    assert e[4] == np.inf and e[5] == 0.0 and np.isnan(e[6])
+
+
+def test_oracle_f64_run_matches_brute_force(oracle_mod):
+    """The double-precision oracle path: flat kernel with unit weights gives exact
+    integer-and-half densities; nearest-higher distances equal the O(n^2) minimum."""
+    rng = np.random.default_rng(21)
+    n = 1500
+    x = rng.uniform(0, 20, (2, n))  # genuinely double coordinates
+    w = np.ones(n)
+    cfg = oracle_mod.Config(ndim=2, dc=1.1, rhoc=2.0, dm=1.7)
+    r = oracle_mod.run(cfg, x, w, dtype=np.float64)
+    d = np.sqrt(((x[:, :, None] - x[:, None, :]) ** 2).sum(0))
+    near = d <= 1.1
+    rho_bf = 1.0 + 0.5 * (near.sum(1) - 1)
+    np.testing.assert_array_equal(r.rho, rho_bf)
+    idx = np.arange(n)
+    higher = (rho_bf[None, :] > rho_bf[:, None]) | (
+        (rho_bf[None, :] == rho_bf[:, None]) & (rho_bf[None, :] > 0) & (idx[None, :] > idx[:, None]))
+    eff = np.where(rho_bf >= 2.0, 1.1, 1.7)
+    dd = np.where(higher & (d <= eff[:, None]), d, np.inf)
+    best = dd.min(1)
+    got = np.where(r.nh >= 0, d[idx, np.maximum(r.nh, 0)], np.inf)
+    np.testing.assert_array_equal(got, best)
+
+
 @pytest.mark.parametrize("n,ndim,nper,ntiles", [(129, 2, 2, 4), (16384, 2, 12, 144),
                                                 (1 << 20, 2, 91, 8281), (1 << 20, 3, 21, 9261),
                                                 (1, 1, 1, 1), (128, 3, 1, 1), (1000, 10, 2, 1024)])
