@@ -10,6 +10,7 @@
 #include <array>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <fstream>
 #include <memory>
 #include <optional>
@@ -42,6 +43,50 @@ struct DeviceDeleter {
   clue_ctx* ctx;
   void operator()(T* p) const {
     if (p) clue_free(ctx, p);
+  }
+};
+
+// Decimal text to T, rounded once (utils/detail/read_csv.hpp:20-30 parses per type).
+template <class T>
+T parse_floating(const std::string& s) {
+  if constexpr (std::is_same_v<T, float>)
+    return std::stof(s);
+  else
+    return std::stod(s);
+}
+
+// The C structs and entry points of each precision (clue_amd.h).
+template <class P>
+using param_t = std::remove_cvref_t<decltype(std::declval<P&>().dc)>;
+
+template <class T>
+struct CApi;
+template <>
+struct CApi<float> {
+  using Params = clue_params;
+  using Points = clue_points;
+  using Results = clue_results;
+  static clue_status run(clue_ctx* c, const Params* p, const Points* pts, Results* r) {
+    return clue_make_clusters(c, p, pts, r);
+  }
+  static clue_status run_batched(clue_ctx* c, const Params* p, const Points* pts,
+                                 const uint32_t* sizes, int32_t n_events, Results* r,
+                                 int32_t* eoc, int64_t* n_clusters) {
+    return clue_make_clusters_batched(c, p, pts, sizes, n_events, r, eoc, n_clusters);
+  }
+};
+template <>
+struct CApi<double> {
+  using Params = clue_params_f64;
+  using Points = clue_points_f64;
+  using Results = clue_results_f64;
+  static clue_status run(clue_ctx* c, const Params* p, const Points* pts, Results* r) {
+    return clue_make_clusters_f64(c, p, pts, r);
+  }
+  static clue_status run_batched(clue_ctx* c, const Params* p, const Points* pts,
+                                 const uint32_t* sizes, int32_t n_events, Results* r,
+                                 int32_t* eoc, int64_t* n_clusters) {
+    return clue_make_clusters_batched_f64(c, p, pts, sizes, n_events, r, eoc, n_clusters);
   }
 };
 
@@ -89,9 +134,10 @@ class FlatKernel {
   FlatKernel(T flat) : m_flat{flat} {
     if (!(flat > T{0})) throw std::invalid_argument("Flat kernel parameter must be positive.");
   }
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.kernel = CLUE_KERNEL_FLAT;
-    p.kparam[0] = static_cast<float>(m_flat);
+    p.kparam[0] = static_cast<detail::param_t<P>>(m_flat);
   }
 
  private:
@@ -107,10 +153,11 @@ class ExponentialKernel {
     if (!(avg > T{0}) || !(amplitude > T{0}))
       throw std::invalid_argument("Exponential kernel parameters must be positive.");
   }
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.kernel = CLUE_KERNEL_EXP;
-    p.kparam[0] = static_cast<float>(m_avg);
-    p.kparam[1] = static_cast<float>(m_amp);
+    p.kparam[0] = static_cast<detail::param_t<P>>(m_avg);
+    p.kparam[1] = static_cast<detail::param_t<P>>(m_amp);
   }
 
  private:
@@ -126,10 +173,11 @@ class GaussianKernel {
     if (!(stddev > T{0}) || !(amplitude > T{0}))
       throw std::invalid_argument("Gaussian kernel parameters must be positive.");
   }
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.kernel = CLUE_KERNEL_GAUSS;
-    p.kparam[0] = static_cast<float>(m_std);
-    p.kparam[1] = static_cast<float>(m_amp);
+    p.kparam[0] = static_cast<detail::param_t<P>>(m_std);
+    p.kparam[1] = static_cast<detail::param_t<P>>(m_amp);
   }
 
  private:
@@ -140,7 +188,8 @@ class GaussianKernel {
 template <std::size_t Ndim, class T = float>
 struct EuclideanMetric {
   static constexpr bool is_metric = true;
-  void fill(clue_params& p) const { p.metric = CLUE_METRIC_EUCLIDEAN; }
+  template <class P>
+  void fill(P& p) const { p.metric = CLUE_METRIC_EUCLIDEAN; }
 };
 template <std::size_t Ndim, class T = float>
 struct WeightedEuclideanMetric {
@@ -150,9 +199,10 @@ struct WeightedEuclideanMetric {
   template <class... V>
     requires(sizeof...(V) == Ndim)
   WeightedEuclideanMetric(V... w) : weights{static_cast<T>(w)...} {}
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.metric = CLUE_METRIC_WEIGHTED_EUCLIDEAN;
-    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(weights[d]);
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(weights[d]);
   }
 };
 template <std::size_t Ndim, class T = float>
@@ -163,20 +213,23 @@ struct PeriodicEuclideanMetric {
   template <class... V>
     requires(sizeof...(V) == Ndim)
   PeriodicEuclideanMetric(V... per) : periods{static_cast<T>(per)...} {}
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.metric = CLUE_METRIC_PERIODIC_EUCLIDEAN;
-    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(periods[d]);
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(periods[d]);
   }
 };
 template <std::size_t Ndim, class T = float>
 struct ManhattanMetric {
   static constexpr bool is_metric = true;
-  void fill(clue_params& p) const { p.metric = CLUE_METRIC_MANHATTAN; }
+  template <class P>
+  void fill(P& p) const { p.metric = CLUE_METRIC_MANHATTAN; }
 };
 template <std::size_t Ndim, class T = float>
 struct ChebyshevMetric {
   static constexpr bool is_metric = true;
-  void fill(clue_params& p) const { p.metric = CLUE_METRIC_CHEBYSHEV; }
+  template <class P>
+  void fill(P& p) const { p.metric = CLUE_METRIC_CHEBYSHEV; }
 };
 template <std::size_t Ndim, class T = float>
 struct WeightedChebyshevMetric {
@@ -186,15 +239,17 @@ struct WeightedChebyshevMetric {
   template <class... V>
     requires(sizeof...(V) == Ndim)
   WeightedChebyshevMetric(V... w) : weights{static_cast<T>(w)...} {}
-  void fill(clue_params& p) const {
+  template <class P>
+  void fill(P& p) const {
     p.metric = CLUE_METRIC_WEIGHTED_CHEBYSHEV;
-    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<float>(weights[d]);
+    for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(weights[d]);
   }
 };
 template <std::size_t Ndim, class T = float>
 struct MahalanobisMetric {
   static constexpr bool is_metric = true;
-  void fill(clue_params& p) const { p.metric = CLUE_METRIC_MAHALANOBIS; }
+  template <class P>
+  void fill(P& p) const { p.metric = CLUE_METRIC_MAHALANOBIS; }
 };
 
 namespace metrics {
@@ -309,11 +364,11 @@ class AssociationMapDevice {
 // the cluster indexes, in one owned buffer or in caller buffers.
 template <std::size_t Ndim, class T = float>
 class PointsHost {
-  static_assert(std::is_same_v<std::remove_cv_t<T>, float>,
-                "the MI355X device path is single precision");
+  static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
+                "points are float or double");
 
  public:
-  using value_type = float;
+  using value_type = T;
 
   PointsHost(Queue&, int32_t n) : m_n{check_n(n)}, m_own((Ndim + 1) * static_cast<std::size_t>(n)),
                                   m_own_idx(static_cast<std::size_t>(n), 0) {
@@ -322,17 +377,17 @@ class PointsHost {
     m_cluster = m_own_idx.data();
   }
   // (Ndim+1) x n SoA input plus an n-int output (the Python binding's layout, Run.hpp:39).
-  PointsHost(Queue&, int32_t n, float* input, int* output) : m_n{check_n(n)} {
+  PointsHost(Queue&, int32_t n, T* input, int* output) : m_n{check_n(n)} {
     for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = input + d * n;
     m_weight = input + Ndim * static_cast<std::size_t>(n);
     m_cluster = output;
   }
-  PointsHost(Queue& q, int32_t n, std::span<float> input, std::span<int> output)
+  PointsHost(Queue& q, int32_t n, std::span<T> input, std::span<int> output)
       : PointsHost(q, n, input.data(), output.data()) {
     if (input.size() < (Ndim + 1) * static_cast<std::size_t>(n) || output.size() < static_cast<std::size_t>(n))
       throw std::invalid_argument("buffers too small for the number of points");
   }
-  PointsHost(Queue&, int32_t n, float* coordinates, float* weights, int* output) : m_n{check_n(n)} {
+  PointsHost(Queue&, int32_t n, T* coordinates, T* weights, int* output) : m_n{check_n(n)} {
     for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = coordinates + d * n;
     m_weight = weights;
     m_cluster = output;
@@ -343,16 +398,16 @@ class PointsHost {
   PointsHost& operator=(PointsHost&&) = default;
 
   int32_t size() const { return m_n; }
-  std::span<float> coords(std::size_t dim) {
+  std::span<T> coords(std::size_t dim) {
     if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
     return {m_coords[dim], static_cast<std::size_t>(m_n)};
   }
-  std::span<const float> coords(std::size_t dim) const {
+  std::span<const T> coords(std::size_t dim) const {
     if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
     return {m_coords[dim], static_cast<std::size_t>(m_n)};
   }
-  std::span<float> weights() { return {m_weight, static_cast<std::size_t>(m_n)}; }
-  std::span<const float> weights() const { return {m_weight, static_cast<std::size_t>(m_n)}; }
+  std::span<T> weights() { return {m_weight, static_cast<std::size_t>(m_n)}; }
+  std::span<const T> weights() const { return {m_weight, static_cast<std::size_t>(m_n)}; }
   std::span<const int> clusterIndexes() const { return {m_cluster, static_cast<std::size_t>(m_n)}; }
   std::span<int> clusterIndexes() { return {m_cluster, static_cast<std::size_t>(m_n)}; }
   bool clustered() const { return m_clustered; }
@@ -363,8 +418,8 @@ class PointsHost {
     for (int32_t i = 0; i < m_n; ++i) m = std::max(m, m_cluster[i]);
     return static_cast<std::size_t>(m + 1);
   }
-  void set_density_uncertainty(std::span<float> u) { m_u = check_span(u).data(); }
-  void set_sigma(std::size_t dim, std::span<float> s) {
+  void set_density_uncertainty(std::span<T> u) { m_u = check_span(u).data(); }
+  void set_sigma(std::size_t dim, std::span<T> s) {
     if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to set_sigma.");
     m_sigma[dim] = check_span(s).data();
   }
@@ -374,11 +429,11 @@ class PointsHost {
   }
   void set_tags(std::vector<std::uint32_t>& tags) { set_tags(std::span<std::uint32_t>(tags)); }
 
-  const float* coord_ptr(std::size_t d) const { return m_coords[d]; }
-  const float* weight_ptr() const { return m_weight; }
+  const T* coord_ptr(std::size_t d) const { return m_coords[d]; }
+  const T* weight_ptr() const { return m_weight; }
   int* cluster_ptr() const { return m_cluster; }
-  const float* sigma_ptr(std::size_t d) const { return m_sigma[d]; }
-  const float* uncertainty_ptr() const { return m_u; }
+  const T* sigma_ptr(std::size_t d) const { return m_sigma[d]; }
+  const T* uncertainty_ptr() const { return m_u; }
   const std::uint32_t* tags_ptr() const { return m_tags; }
 
  private:
@@ -386,18 +441,18 @@ class PointsHost {
     if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
     return n;
   }
-  std::span<float> check_span(std::span<float> s) const {
+  std::span<T> check_span(std::span<T> s) const {
     if (s.size() < static_cast<std::size_t>(m_n)) throw std::invalid_argument("array too short");
     return s;
   }
   int32_t m_n;
-  std::vector<float> m_own;
+  std::vector<T> m_own;
   std::vector<int> m_own_idx;
-  std::array<float*, Ndim> m_coords{};
-  float* m_weight = nullptr;
+  std::array<T*, Ndim> m_coords{};
+  T* m_weight = nullptr;
   int* m_cluster = nullptr;
-  std::array<float*, Ndim> m_sigma{};
-  float* m_u = nullptr;
+  std::array<T*, Ndim> m_sigma{};
+  T* m_u = nullptr;
   std::uint32_t* m_tags = nullptr;
   bool m_clustered = false;
 };
@@ -407,31 +462,31 @@ class PointsHost {
 // uncertainty and tags. All device memory, owned.
 template <std::size_t Ndim, class T = float>
 class PointsDevice {
-  static_assert(std::is_same_v<std::remove_cv_t<T>, float>,
-                "the MI355X device path is single precision");
+  static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
+                "points are float or double");
 
  public:
   struct View {
-    std::array<float*, Ndim> m_coords;
-    float* m_weight;
+    std::array<T*, Ndim> m_coords;
+    T* m_weight;
     int32_t* m_cluster_index;
     int32_t* m_is_seed;
-    float* m_rho;
+    T* m_rho;
     int32_t* m_nearest_higher;
-    float* m_delta;
+    T* m_delta;
     int32_t m_n;
     std::span<int32_t> cluster_index() const { return {m_cluster_index, std::size_t(m_n)}; }
     std::span<int32_t> is_seed() const { return {m_is_seed, std::size_t(m_n)}; }
-    std::span<float> rho() const { return {m_rho, std::size_t(m_n)}; }
+    std::span<T> rho() const { return {m_rho, std::size_t(m_n)}; }
     std::span<int32_t> nearest_higher() const { return {m_nearest_higher, std::size_t(m_n)}; }
-    std::span<float> delta() const { return {m_delta, std::size_t(m_n)}; }
+    std::span<T> delta() const { return {m_delta, std::size_t(m_n)}; }
     int32_t size() const { return m_n; }
   };
 
   PointsDevice(Queue& q, int32_t n) : m_q{q}, m_n{n} {
     if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
     const std::size_t nn = static_cast<std::size_t>(n);
-    m_block = alloc<float>((Ndim + 3) * nn);
+    m_block = alloc<T>((Ndim + 3) * nn);
     m_ints = alloc<int32_t>(3 * nn);
     for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = m_block.get() + d * nn;
     m_view.m_weight = m_block.get() + Ndim * nn;
@@ -444,45 +499,46 @@ class PointsDevice {
   }
   int32_t size() const { return m_n; }
   const View& view() const { return m_view; }
-  std::span<float> coords(std::size_t d) const {
+  std::span<T> coords(std::size_t d) const {
     if (d >= Ndim) throw std::out_of_range("Dimension out of range in call to coords.");
     return {m_view.m_coords[d], std::size_t(m_n)};
   }
-  std::span<float> weights() const { return {m_view.m_weight, std::size_t(m_n)}; }
+  std::span<T> weights() const { return {m_view.m_weight, std::size_t(m_n)}; }
   std::span<int32_t> clusterIndexes() const { return m_view.cluster_index(); }
 
-  void set_sigma(std::size_t dim, std::span<float> device_sigma) {
+  void set_sigma(std::size_t dim, std::span<T> device_sigma) {
     if (dim >= Ndim) throw std::out_of_range("Dimension out of range in call to set_sigma.");
     m_sigma[dim] = device_sigma.data();
   }
-  void set_density_uncertainty(std::span<float> device_u) { m_u = device_u.data(); }
+  void set_density_uncertainty(std::span<T> device_u) { m_u = device_u.data(); }
   void set_tags(std::span<std::uint32_t> device_tags) { m_tags = device_tags.data(); }
 
   // Copies host sigma / uncertainty / tags into device buffers owned by these points.
-  void upload_optionals(const PointsHost<Ndim, float>& h) {
+  void upload_optionals(const PointsHost<Ndim, T>& h) {
     const std::size_t nn = std::size_t(m_n);
     if (h.sigma_ptr(0)) {
-      m_sig_own = alloc<float>(Ndim * nn);
+      m_sig_own = alloc<T>(Ndim * nn);
       for (std::size_t d = 0; d < Ndim; ++d) {
         if (!h.sigma_ptr(d)) throw std::invalid_argument("sigma must be set in every dimension");
-        detail::check(clue_memcpy_h2d(m_q.ctx(), m_sig_own.get() + d * nn, h.sigma_ptr(d), 4 * nn));
+        detail::check(clue_memcpy_h2d(m_q.ctx(), m_sig_own.get() + d * nn, h.sigma_ptr(d), sizeof(T) * nn));
         m_sigma[d] = m_sig_own.get() + d * nn;
       }
     }
     if (h.uncertainty_ptr()) {
-      m_u_own = alloc<float>(nn);
-      detail::check(clue_memcpy_h2d(m_q.ctx(), m_u_own.get(), h.uncertainty_ptr(), 4 * nn));
+      m_u_own = alloc<T>(nn);
+      detail::check(clue_memcpy_h2d(m_q.ctx(), m_u_own.get(), h.uncertainty_ptr(), sizeof(T) * nn));
       m_u = m_u_own.get();
     }
     if (h.tags_ptr()) {
       m_tags_own = alloc<std::uint32_t>(nn);
-      detail::check(clue_memcpy_h2d(m_q.ctx(), m_tags_own.get(), h.tags_ptr(), 4 * nn));
+      detail::check(
+          clue_memcpy_h2d(m_q.ctx(), m_tags_own.get(), h.tags_ptr(), sizeof(std::uint32_t) * nn));
       m_tags = m_tags_own.get();
     }
   }
 
-  clue_points c_points(int32_t offset = 0, int32_t n = -1) const {
-    clue_points p{};
+  typename detail::CApi<T>::Points c_points(int32_t offset = 0, int32_t n = -1) const {
+    typename detail::CApi<T>::Points p{};
     p.n = n < 0 ? m_n : n;
     for (std::size_t d = 0; d < Ndim; ++d) {
       p.coords[d] = m_view.m_coords[d] + offset;
@@ -493,8 +549,8 @@ class PointsDevice {
     p.tags = m_tags ? m_tags + offset : nullptr;
     return p;
   }
-  clue_results c_results() const {
-    clue_results r{};
+  typename detail::CApi<T>::Results c_results() const {
+    typename detail::CApi<T>::Results r{};
     r.cluster_index = m_view.m_cluster_index;
     r.is_seed = m_view.m_is_seed;
     r.rho = m_view.m_rho;
@@ -523,76 +579,124 @@ class PointsDevice {
   }
   Queue m_q;
   int32_t m_n;
-  std::unique_ptr<float, detail::DeviceDeleter<float>> m_block;
+  std::unique_ptr<T, detail::DeviceDeleter<T>> m_block;
   std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_ints;
-  std::unique_ptr<float, detail::DeviceDeleter<float>> m_sig_own, m_u_own;
+  std::unique_ptr<T, detail::DeviceDeleter<T>> m_sig_own, m_u_own;
   std::unique_ptr<std::uint32_t, detail::DeviceDeleter<std::uint32_t>> m_tags_own;
   View m_view{};
-  std::array<float*, Ndim> m_sigma{};
-  float* m_u = nullptr;
+  std::array<T*, Ndim> m_sigma{};
+  T* m_u = nullptr;
   std::uint32_t* m_tags = nullptr;
   bool m_clustered = false;
 };
 
 // copyToDevice / copyToHost (data_structures/detail/PointsConversion.hpp:21-111).
-template <std::size_t Ndim>
-void copyToDevice(Queue& q, PointsDevice<Ndim, float>& d, const PointsHost<Ndim, float>& h) {
+template <std::size_t Ndim, class T>
+void copyToDevice(Queue& q, PointsDevice<Ndim, T>& d, const PointsHost<Ndim, T>& h) {
   if (d.size() != h.size()) throw std::invalid_argument("host and device sizes differ");
   const std::size_t nn = std::size_t(h.size());
   for (std::size_t k = 0; k < Ndim; ++k)
-    detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_coords[k], h.coord_ptr(k), 4 * nn));
-  detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_weight, h.weight_ptr(), 4 * nn));
+    detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_coords[k], h.coord_ptr(k), sizeof(T) * nn));
+  detail::check(clue_memcpy_h2d(q.ctx(), d.view().m_weight, h.weight_ptr(), sizeof(T) * nn));
   d.upload_optionals(h);
 }
-template <std::size_t Ndim>
-void copyToHost(Queue& q, PointsHost<Ndim, float>& h, const PointsDevice<Ndim, float>& d) {
+template <std::size_t Ndim, class T>
+void copyToHost(Queue& q, PointsHost<Ndim, T>& h, const PointsDevice<Ndim, T>& d) {
   if (d.size() != h.size()) throw std::invalid_argument("host and device sizes differ");
   detail::check(clue_memcpy_d2h(q.ctx(), h.cluster_ptr(), d.view().m_cluster_index,
-                                4 * std::size_t(h.size())));
+                                sizeof(int32_t) * std::size_t(h.size())));
   h.mark_clustered();
 }
 
 inline AssociationMapHost get_clusters(std::span<const int32_t> cluster_index) {
   return AssociationMapHost::from_associations(cluster_index);
 }
-template <std::size_t Ndim>
-AssociationMapHost get_clusters(const PointsHost<Ndim, float>& h) {
+template <std::size_t Ndim, class T>
+AssociationMapHost get_clusters(const PointsHost<Ndim, T>& h) {
   return get_clusters(h.clusterIndexes());
 }
 // get_clusters(queue, d_points) (utils/detail/get_clusters.hpp:44-48): built on the device.
-template <std::size_t Ndim>
-AssociationMapDevice get_clusters(Queue& q, const PointsDevice<Ndim, float>& d) {
+template <std::size_t Ndim, class T>
+AssociationMapDevice get_clusters(Queue& q, const PointsDevice<Ndim, T>& d) {
   return AssociationMapDevice(q.ctx(), d.clusterIndexes().data(), d.size());
+}
+
+// ---- silhouette (utils/scores.hpp, detail/scores.hpp:25-99) -------------------------------
+// Score of one clustered point: (b - a) / max(a, b), with a the mean Euclidean
+// distance to the other members of its cluster and b the smallest mean distance
+// to the members of another cluster.
+template <std::size_t Ndim, class T>
+T silhouette(const AssociationMapHost& clusters, const PointsHost<Ndim, T>& h, int point) {
+  auto dist = [&](int i, int j) {
+    T s = T(0);
+    for (std::size_t d = 0; d < Ndim; ++d) {
+      const T dx = h.coords(d)[i] - h.coords(d)[j];
+      s += dx * dx;
+    }
+    return std::sqrt(s);
+  };
+  const int c = h.clusterIndexes()[point];
+  T a = T(0);
+  for (int j : clusters[c])
+    if (j != point) a += dist(point, j);
+  a /= static_cast<T>(clusters.count(c) - 1);
+  T b = std::numeric_limits<T>::max();
+  for (int k = 0; k < static_cast<int>(clusters.size()); ++k) {
+    if (k == c) continue;
+    T m = T(0);
+    for (int j : clusters[k]) m += dist(point, j);
+    b = std::min(b, m / static_cast<T>(clusters.count(k)));
+  }
+  return (b - a) / std::max(a, b);
+}
+template <std::size_t Ndim, class T>
+T silhouette(const PointsHost<Ndim, T>& h, int point) {
+  return silhouette(get_clusters(h), h, point);
+}
+// Mean over the clustered points of clusters with at least two members.
+template <std::size_t Ndim, class T>
+T silhouette(const PointsHost<Ndim, T>& h) {
+  const AssociationMapHost clusters = get_clusters(h);
+  T sum = T(0);
+  std::size_t n = 0;
+  for (int i = 0; i < h.size(); ++i) {
+    const int c = h.clusterIndexes()[i];
+    if (c < 0 || clusters.count(c) < 2) continue;
+    sum += silhouette(clusters, h, i);
+    ++n;
+  }
+  return sum / static_cast<T>(n);
 }
 
 // ---- Clusterer (core/Clusterer.hpp:37-312) -----------------------------------------------
 template <std::size_t Ndim, class DataType = float>
 class Clusterer {
-  static_assert(std::is_same_v<DataType, float>, "the MI355X device path is single precision");
+  static_assert(std::is_same_v<DataType, float> || std::is_same_v<DataType, double>,
+                "the device path computes in value_type or double");
   static_assert(Ndim >= 1 && Ndim <= 10, "This library only works up to 10 dimensions");
 
  public:
-  using value_type = float;
+  using value_type = DataType;
 
-  Clusterer(float dc, float rhoc, std::optional<float> dm = std::nullopt,
-            std::optional<float> seed_dc = std::nullopt)
+  Clusterer(value_type dc, value_type rhoc, std::optional<value_type> dm = std::nullopt,
+            std::optional<value_type> seed_dc = std::nullopt)
       : m_q{std::make_shared<Queue>(0)} {
     setParameters(dc, rhoc, dm, seed_dc);
   }
-  Clusterer(Queue& q, float dc, float rhoc, std::optional<float> dm = std::nullopt,
-            std::optional<float> seed_dc = std::nullopt)
+  Clusterer(Queue& q, value_type dc, value_type rhoc, std::optional<value_type> dm = std::nullopt,
+            std::optional<value_type> seed_dc = std::nullopt)
       : m_q{std::make_shared<Queue>(q)} {
     setParameters(dc, rhoc, dm, seed_dc);
   }
 
   // core/detail/Clusterer.hpp:75-89
-  void setParameters(float dc, float rhoc, std::optional<float> dm = std::nullopt,
-                     std::optional<float> seed_dc = std::nullopt) {
+  void setParameters(value_type dc, value_type rhoc, std::optional<value_type> dm = std::nullopt,
+                     std::optional<value_type> seed_dc = std::nullopt) {
     m_dc = dc;
     m_rhoc = rhoc;
     m_dm = dm.value_or(dc);
     m_seed_dc = seed_dc.value_or(dc);
-    if (!(m_dc > 0.f) || !(m_rhoc >= 0.f) || !(m_dm > 0.f) || !(m_seed_dc > 0.f))
+    if (!(m_dc > 0) || !(m_rhoc >= 0) || !(m_dm > 0) || !(m_seed_dc > 0))
       throw std::invalid_argument("Invalid clustering parameters. The parameters must be positive.");
   }
 
@@ -610,67 +714,67 @@ class Clusterer {
     m_wrapped = {static_cast<uint8_t>(wrapped)...};
   }
 
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, const Metric& metric = Metric{},
-                     const Kernel& kernel = Kernel{.5f}) {
-    PointsDevice<Ndim, float> d(q, h.size());
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(Queue& q, PointsHost<Ndim, value_type>& h, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{value_type(.5)}) {
+    PointsDevice<Ndim, value_type> d(q, h.size());
     make_clusters(q, h, d, metric, kernel);
   }
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(PointsHost<Ndim, float>& h, const Metric& metric = Metric{},
-                     const Kernel& kernel = Kernel{.5f}) {
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(PointsHost<Ndim, value_type>& h, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{value_type(.5)}) {
     Queue q(0);
     make_clusters(q, h, metric, kernel);
   }
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, PointsDevice<Ndim, float>& d,
-                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{.5f}) {
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(Queue& q, PointsHost<Ndim, value_type>& h, PointsDevice<Ndim, value_type>& d,
+                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{value_type(.5)}) {
     copyToDevice(q, d, h);
     make_clusters(q, d, metric, kernel);
     copyToHost(q, h, d);
   }
   // Device-resident per event (core/detail/Clusterer.hpp:143-151).
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(Queue& q, PointsDevice<Ndim, float>& d, const Metric& metric = Metric{},
-                     const Kernel& kernel = Kernel{.5f}) {
-    clue_params p = params(metric, kernel);
-    clue_points pts = d.c_points();
-    clue_results res = d.c_results();
-    detail::check(clue_make_clusters(q.ctx(), &p, &pts, &res));
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(Queue& q, PointsDevice<Ndim, value_type>& d, const Metric& metric = Metric{},
+                     const Kernel& kernel = Kernel{value_type(.5)}) {
+    auto p = params(metric, kernel);
+    auto pts = d.c_points();
+    auto res = d.c_results();
+    detail::check(detail::CApi<value_type>::run(q.ctx(), &p, &pts, &res));
     m_last = q;
     m_event_of_cluster.clear();
     m_eoc_dev.reset();
     d.mark_clustered();
   }
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(Queue& q, PointsHost<Ndim, float>& h, PointsDevice<Ndim, float>& d,
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(Queue& q, PointsHost<Ndim, value_type>& h, PointsDevice<Ndim, value_type>& d,
                      std::span<const uint32_t> sizes, const Metric& metric = Metric{},
-                     const Kernel& kernel = Kernel{.5f}) {
+                     const Kernel& kernel = Kernel{value_type(.5)}) {
     copyToDevice(q, d, h);
     make_clusters(q, d, sizes, metric, kernel);
     copyToHost(q, h, d);
   }
   // Batched (core/detail/Clusterer.hpp:171-183, 286-350); every event is tiled alone.
-  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, float>,
-            concepts::convolutional_kernel Kernel = FlatKernel<float>>
-  void make_clusters(Queue& q, PointsDevice<Ndim, float>& d, std::span<const uint32_t> sizes,
-                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{.5f}) {
-    clue_params p = params(metric, kernel);
-    clue_points pts = d.c_points();
-    clue_results res = d.c_results();
+  template <concepts::distance_metric Metric = EuclideanMetric<Ndim, value_type>,
+            concepts::convolutional_kernel Kernel = FlatKernel<value_type>>
+  void make_clusters(Queue& q, PointsDevice<Ndim, value_type>& d, std::span<const uint32_t> sizes,
+                     const Metric& metric = Metric{}, const Kernel& kernel = Kernel{value_type(.5)}) {
+    auto p = params(metric, kernel);
+    auto pts = d.c_points();
+    auto res = d.c_results();
     void* eoc = nullptr;
     detail::check(clue_malloc(q.ctx(), 4 * std::size_t(d.size()), &eoc));
     m_eoc_dev.reset(static_cast<int32_t*>(eoc));
     m_eoc_dev.get_deleter().ctx = q.ctx();
     int64_t nclusters = 0;
-    clue_status st = clue_make_clusters_batched(q.ctx(), &p, &pts, sizes.data(),
-                                                static_cast<int32_t>(sizes.size()), &res,
-                                                m_eoc_dev.get(), &nclusters);
+    clue_status st = detail::CApi<value_type>::run_batched(
+        q.ctx(), &p, &pts, sizes.data(), static_cast<int32_t>(sizes.size()), &res,
+        m_eoc_dev.get(), &nclusters);
     if (st == CLUE_OK) {
       m_event_of_cluster.assign(std::size_t(nclusters), 0);
       if (nclusters > 0)
@@ -692,16 +796,16 @@ class Clusterer {
   }
   // getClusters (core/detail/Clusterer.hpp:205-217): cluster -> points, on the host
   // for host points and on the device for device points.
-  AssociationMapHost getClusters(const PointsHost<Ndim, float>& h) const { return get_clusters(h); }
-  AssociationMapDevice getClusters(Queue& q, const PointsDevice<Ndim, float>& d) const {
+  AssociationMapHost getClusters(const PointsHost<Ndim, value_type>& h) const { return get_clusters(h); }
+  AssociationMapDevice getClusters(Queue& q, const PointsDevice<Ndim, value_type>& d) const {
     return get_clusters(q, d);
   }
   // getSampleAssociations (core/detail/Clusterer.hpp:219-241): event -> clusters of the
   // last batched run, on the host or on the device.
-  AssociationMapHost getSampleAssociations(Queue&, const PointsHost<Ndim, float>&) const {
+  AssociationMapHost getSampleAssociations(Queue&, const PointsHost<Ndim, value_type>&) const {
     return AssociationMapHost::from_associations(m_event_of_cluster);
   }
-  AssociationMapDevice getSampleAssociations(Queue& q, const PointsDevice<Ndim, float>&) const {
+  AssociationMapDevice getSampleAssociations(Queue& q, const PointsDevice<Ndim, value_type>&) const {
     if (!m_eoc_dev) throw std::runtime_error("No batched run to associate. Run make_clusters first.");
     return AssociationMapDevice(q.ctx(), m_eoc_dev.get(),
                                 static_cast<int32_t>(m_event_of_cluster.size()));
@@ -709,8 +813,9 @@ class Clusterer {
 
  private:
   template <class Metric, class Kernel>
-  clue_params params(const Metric& metric, const Kernel& kernel) const {
-    clue_params p{};
+  typename detail::CApi<value_type>::Params params(const Metric& metric,
+                                                   const Kernel& kernel) const {
+    typename detail::CApi<value_type>::Params p{};
     p.ndim = static_cast<int32_t>(Ndim);
     p.dc = m_dc;
     p.rhoc = m_rhoc;
@@ -724,7 +829,7 @@ class Clusterer {
 
   std::shared_ptr<Queue> m_q;
   std::optional<Queue> m_last;
-  float m_dc{}, m_rhoc{}, m_dm{}, m_seed_dc{};
+  value_type m_dc{}, m_rhoc{}, m_dm{}, m_seed_dc{};
   std::array<uint8_t, Ndim> m_wrapped{};
   std::vector<int32_t> m_event_of_cluster;
   std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_eoc_dev{nullptr, {nullptr}};
@@ -738,15 +843,15 @@ PointsHost<Ndim, T> read_csv(Queue& q, const std::string& path) {
   if (!f) throw std::runtime_error("Could not open file " + path);
   std::string line;
   std::getline(f, line);
-  std::vector<std::array<float, Ndim + 1>> rows;
+  std::vector<std::array<T, Ndim + 1>> rows;
   while (std::getline(f, line)) {
     if (line.empty()) continue;
     std::stringstream ss(line);
-    std::array<float, Ndim + 1> r{};
+    std::array<T, Ndim + 1> r{};
     std::string cell;
     for (std::size_t k = 0; k < Ndim + 1; ++k) {
       if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
-      r[k] = std::stof(cell);
+      r[k] = detail::parse_floating<T>(cell);
     }
     rows.push_back(r);
   }
@@ -767,16 +872,16 @@ PointsHost<Ndim, T> read_output(Queue& q, const std::string& path) {
   if (!f) throw std::runtime_error("Could not open file: " + path);
   std::string line;
   std::getline(f, line);
-  std::vector<std::array<float, Ndim + 1>> rows;
+  std::vector<std::array<T, Ndim + 1>> rows;
   std::vector<int32_t> ids;
   while (std::getline(f, line)) {
     if (line.empty()) continue;
     std::stringstream ss(line);
-    std::array<float, Ndim + 1> r{};
+    std::array<T, Ndim + 1> r{};
     std::string cell;
     for (std::size_t k = 0; k < Ndim + 1; ++k) {
       if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
-      r[k] = std::stof(cell);
+      r[k] = detail::parse_floating<T>(cell);
     }
     if (!std::getline(ss, cell, ',')) throw std::runtime_error("Malformed line in " + path);
     rows.push_back(r);

@@ -201,6 +201,70 @@ static void test_device_points_and_clusters() {
   std::remove(out.c_str());
 }
 
+static void test_precisions_and_silhouette() {
+  // tests/test_clustering.cpp:14-57, 94-130: single and double precision on the
+  // benchmark and sissa fixtures, judged by the silhouette score.
+  auto q = clue::get_queue(0u);
+  for (const char* name : {"data_1024.csv", "data_4096.csv"}) {
+    clue::PointsHost<2> hf = clue::read_csv<2, float>(q, data_dir + "/" + name);
+    clue::PointsDevice<2> df(q, hf.size());
+    clue::Clusterer<2> af(q, 1.5f, 10.f, 1.5f);
+    af.make_clusters(q, hf, df);
+    CHECK(clue::silhouette(hf) >= 0.9f);
+
+    clue::PointsHost<2, double> hd = clue::read_csv<2, double>(q, data_dir + "/" + name);
+    clue::PointsDevice<2, double> dd(q, hd.size());
+    clue::Clusterer<2, double> ad(q, 1.5, 10., 1.5);
+    ad.make_clusters(q, hd, dd);
+    CHECK(clue::silhouette(hd) >= 0.9);
+    CHECK(hd.n_clusters() == hf.n_clusters());
+    CHECK(ad.getSeeds().size() == hd.n_clusters());
+  }
+  {
+    clue::PointsHost<2, double> h = clue::read_csv<2, double>(q, data_dir + "/sissa_1000.csv");
+    clue::PointsDevice<2, double> d(q, h.size());
+    clue::Clusterer<2, double> algo(q, 20., 10., 20.);
+    algo.make_clusters(q, h, d);
+    CHECK(clue::silhouette(h) >= 0.5);
+  }
+  {
+    // Double batched: 3 copies of data_1024, each event clustered alone.
+    auto one = clue::read_csv<2, double>(q, data_dir + "/data_1024.csv");
+    clue::Clusterer<2, double> algo(q, 1.3, 10., 1.3);
+    algo.make_clusters(q, one);
+    const auto truth = one.n_clusters();
+    clue::PointsHost<2, double> h(q, 3 * 1024);
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 1024; ++i) {
+        h.coords(0)[k * 1024 + i] = one.coords(0)[i];
+        h.coords(1)[k * 1024 + i] = one.coords(1)[i];
+        h.weights()[k * 1024 + i] = one.weights()[i];
+      }
+    clue::PointsDevice<2, double> d(q, 3 * 1024);
+    std::vector<uint32_t> sizes(3, 1024);
+    algo.make_clusters(q, h, d, sizes, clue::EuclideanMetric<2, double>{},
+                       clue::GaussianKernel<double>(1.0, 1.0));
+    clue::PointsHost<2, double> g(q, 1024);
+    for (int i = 0; i < 1024; ++i) {
+      g.coords(0)[i] = one.coords(0)[i];
+      g.coords(1)[i] = one.coords(1)[i];
+      g.weights()[i] = one.weights()[i];
+    }
+    algo.make_clusters(q, g, clue::EuclideanMetric<2, double>{},
+                       clue::GaussianKernel<double>(1.0, 1.0));
+    const int per = static_cast<int>(g.n_clusters());
+    bool same = true;
+    for (int k = 0; k < 3; ++k)
+      for (int i = 0; i < 1024; ++i) {
+        const int a = g.clusterIndexes()[i];
+        const int b = h.clusterIndexes()[k * 1024 + i];
+        same = same && (a < 0 ? b == -1 : b == a + k * per);
+      }
+    CHECK(same);
+    CHECK(truth > 0);
+  }
+}
+
 int main(int argc, char** argv) {
   data_dir = argc > 1 ? argv[1] : "tests/golden/reference_data";
   try {
@@ -210,6 +274,7 @@ int main(int argc, char** argv) {
     test_periodic();
     test_batched();
     test_device_points_and_clusters();
+    test_precisions_and_silhouette();
   } catch (const std::exception& e) {
     std::printf("FAIL exception: %s\n", e.what());
     ++g_fail;
