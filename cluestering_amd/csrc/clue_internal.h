@@ -18,7 +18,7 @@ int64_t radix_hist_elems(int64_t n);
 void radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n, int key_bits,
                       int32_t* hist, int32_t* scan_temp, bool* in_first, hipStream_t s);
 
-// ---- stages (clue_stages.hip) ----
+// ---- stages (clue_stages_impl.h, instantiated in clue_stages_f32.hip / _f64.hip) ----
 
 // Integer types of the same width as the point type: ordered encodings of
 // coordinates (extremes) and of densities (nearest-higher keys).

@@ -14,6 +14,11 @@
 //   density   core/detail/ClusteringKernels.hpp:29-140
 //   NH        core/detail/ClusteringKernels.hpp:142-291
 //   seeds     core/detail/ClusteringKernels.hpp:293-343
+//
+// Included by one translation unit per point type (clue_stages_f32.hip,
+// clue_stages_f64.hip), which compile in parallel.
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -214,14 +219,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs<T> a, const 
 }
 
 // Work items: ceil(count / 64) per tile.
-__global__ void k_item_count(const int32_t* __restrict__ toff, int n_tiles,
+static __global__ void k_item_count(const int32_t* __restrict__ toff, int n_tiles,
                              int32_t* __restrict__ icount) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_tiles) return;
   icount[k] = (toff[k + 1] - toff[k] + kWave - 1) / kWave;
 }
 
-__global__ void k_item_fill(const int32_t* __restrict__ toff, const int32_t* __restrict__ ioff,
+static __global__ void k_item_fill(const int32_t* __restrict__ toff, const int32_t* __restrict__ ioff,
                             int n_tiles, const EventDesc* __restrict__ events, int n_events,
                             Item* __restrict__ items) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -909,7 +914,7 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // where a chain stays among neighbouring tiles of one event. An event occupies
 // the same range of positions in input and sorted order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
+static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
                                                    const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ link,
@@ -1049,11 +1054,6 @@ static void launch_neighbour(const NeighbourLaunch<T>& L, bool density) {
   }
 }
 
-const char* stage_name(int s) {
-  static const char* names[kNumStages] = {"extremes", "tiles", "density", "nearest_higher",
-                                          "seeds", "labels"};
-  return (s >= 0 && s < kNumStages) ? names[s] : "?";
-}
 
 template <class T>
 void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const BatchBuffersT<T>& bb,
@@ -1124,12 +1124,5 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
                      out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
-
-template void launch_pipeline<float>(const RunParamsT<float>&, const PointArraysT<float>&,
-                                     const BatchBuffersT<float>&, const ResultArraysT<float>&,
-                                     hipStream_t, const StageHook*);
-template void launch_pipeline<double>(const RunParamsT<double>&, const PointArraysT<double>&,
-                                      const BatchBuffersT<double>&, const ResultArraysT<double>&,
-                                      hipStream_t, const StageHook*);
 
 }  // namespace clue_dev
