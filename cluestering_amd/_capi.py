@@ -104,7 +104,8 @@ class Tuning(C.Structure):
         ("group_queries", C.c_int32),
         ("xcd_chunk_log2", C.c_int32),
         ("block_waves", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("nh_plain_scan", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

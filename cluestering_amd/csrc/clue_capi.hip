@@ -405,6 +405,7 @@ clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning) {
     ctx->tuning.group_queries = tuning->group_queries ? 1 : 0;
     ctx->tuning.xcd_chunk_log2 = tuning->xcd_chunk_log2;
     ctx->tuning.block_waves = tuning->block_waves ? tuning->block_waves : 4;
+    ctx->tuning.nh_plain_scan = tuning->nh_plain_scan ? 1 : 0;
   }
   return CLUE_OK;
 }

@@ -94,6 +94,15 @@ struct BatchBuffersT {
   int32_t* seeds;      // [n_total]
   SortedArraysT<T> sorted;
   int64_t sorted_stride;  // column stride of sorted.x[] and sorted.sigma[] blocks
+  // Nearest-higher block layout (NULL when not used): each tile's candidates in
+  // Morton order from block ceil(toff[t] / kNhBlock) + t on, padded to whole
+  // blocks with key 0 (never higher) and index -1.
+  T* hx;             // [D][h_stride] coordinates
+  uint64_t* hkey;    // [h_stride] ordering keys
+  int32_t* hidx;     // [h_stride] sorted position of the candidate
+  T* hbox;           // [h_blocks][2][D] bounding box (lower corner, upper corner)
+  uint64_t* hbkey;   // [h_blocks] largest key of the block
+  int64_t h_stride;  // slots = h_blocks * kNhBlock
 };
 using BatchBuffers = BatchBuffersT<float>;
 

@@ -63,7 +63,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_NUM
 };
 
 struct Workspace {

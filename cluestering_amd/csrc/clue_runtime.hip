@@ -142,6 +142,20 @@ clue_status validate(const P* p) {
   return CLUE_OK;
 }
 
+// Whether the nearest-higher stage runs over the Morton-ordered block layout:
+// it needs the Morton order of the query grouping, candidates visited in any
+// order (unique tags: the default ones), and a metric with a lower bound over a
+// bounding box (not Mahalanobis; weighted metrics with weights >= 0).
+template <class P, class Pts>
+bool nh_blocks_enabled(const P* p, const Pts* pts, const BatchPlan& bp) {
+  if (bp.tuning.nh_plain_scan || !bp.tuning.group_queries || pts->tags != nullptr) return false;
+  if (p->metric == CLUE_METRIC_MAHALANOBIS) return false;
+  if (p->metric == CLUE_METRIC_WEIGHTED_EUCLIDEAN || p->metric == CLUE_METRIC_WEIGHTED_CHEBYSHEV)
+    for (int d = 0; d < p->ndim; ++d)
+      if (!(p->mparam[d] >= 0)) return false;
+  return true;
+}
+
 template <class P, class Pts>
 RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPlan& bp) {
   using T = value_t<P>;
@@ -150,6 +164,7 @@ RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPl
   RunParamsT<T> rp{};
   rp.xcd_chunk_log2 = bp.tuning.xcd_chunk_log2;
   rp.block_waves = bp.tuning.block_waves > 0 ? bp.tuning.block_waves : 4;
+  rp.nh_blocks = nh_blocks_enabled(p, pts, bp) ? 1 : 0;
   rp.D = p->ndim;
   rp.kernel = p->kernel;
   rp.metric = p->metric;
@@ -235,6 +250,9 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
       std::max({scan_temp_elems(n), scan_temp_elems(hist), scan_temp_elems(NT)}) + 16;
   const int64_t stride = (n + 63) / 64 * 64;
   const bool has_sigma = pts->sigma[0] != nullptr;
+  const bool nhb = nh_blocks_enabled(p, pts, bp);
+  const int64_t h_blocks = nhb ? (n + kNhBlock - 1) / kNhBlock + NT + 1 : 0;
+  const int64_t h_slots = h_blocks * kNhBlock;
   struct Req {
     Slot s;
     size_t bytes;
@@ -258,6 +276,13 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
       {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc) ? sizeof(int32_t) * n : 0},
       {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_QKEY, bp.tuning.group_queries ? sizeof(uint32_t) * n : 0},
+      {S_HX, sizeof(T) * h_slots * D},
+      {S_HKEY, sizeof(uint64_t) * h_slots},
+      {S_HIDX, sizeof(int32_t) * h_slots},
+      // Block boxes and keys are read in groups of up to 4 from a tile's first
+      // block: 4 spare entries keep the last group in bounds.
+      {S_HBOX, nhb ? sizeof(T) * (h_blocks + 4) * 2 * D : 0},
+      {S_HBKEY, nhb ? sizeof(uint64_t) * (h_blocks + 4) : 0},
       {S_SRHO, sizeof(T) * n},
       {S_SKEY, sizeof(uint64_t) * n},
       {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(T) * n : 0},
@@ -318,6 +343,12 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
   b.n_seeds = ws.buf[S_NSEEDS].as<int32_t>();
   b.seeds = ws.buf[S_SEEDS].as<int32_t>();
   b.sorted_stride = stride;
+  b.hx = nhb ? ws.buf[S_HX].as<T>() : nullptr;
+  b.hkey = nhb ? ws.buf[S_HKEY].as<uint64_t>() : nullptr;
+  b.hidx = nhb ? ws.buf[S_HIDX].as<int32_t>() : nullptr;
+  b.hbox = nhb ? ws.buf[S_HBOX].as<T>() : nullptr;
+  b.hbkey = nhb ? ws.buf[S_HBKEY].as<uint64_t>() : nullptr;
+  b.h_stride = h_slots;
   for (int d = 0; d < kMaxDim; ++d) {
     b.sorted.x[d] = d < D ? ws.buf[S_SX].as<T>() + stride * d : nullptr;
     b.sorted.sigma[d] = (has_sigma && d < D) ? ws.buf[S_SSIG].as<T>() + stride * d : nullptr;

@@ -258,14 +258,13 @@ static __global__ void k_item_fill(const int32_t* __restrict__ toff, const int32
 // For square-root metrics the value returned is the sum under the root.
 // ---------------------------------------------------------------------------
 template <int D, int M, class T>
-__device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
-                                          const T* __restrict__ sx, int64_t cs, int j,
-                                          const T* __restrict__ ssig, const RunParamsT<T>& rp) {
+__device__ __forceinline__ T metric_pair(const T (&xi)[D], const T (&si)[D], const T (&xj)[D],
+                                         const T (&sj)[D], const RunParamsT<T>& rp) {
   if constexpr (M == M_EUCLID) {
     T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const T dx = xi[d] - sx[d * cs + j];
+      const T dx = xi[d] - xj[d];
       s = (d == D - 1) ? dx * dx : dx * dx + s;  // t + 0 == t for t = dx*dx >= +0
     }
     return s;
@@ -273,7 +272,7 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T s = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const T dx = xi[d] - sx[d * cs + j];
+      const T dx = xi[d] - xj[d];
       s = rp.mparam[d] * dx * dx + s;
     }
     return s;
@@ -281,7 +280,7 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
+      const T a = __builtin_fabs(xi[d] - xj[d]);
       const T pa = rp.mparam[d] - a;
       const T m = (pa < a) ? pa : a;  // std::min(a, P - a)
       s = (d == D - 1) ? m * m : m * m + s;
@@ -291,7 +290,7 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T s;
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
+      const T a = __builtin_fabs(xi[d] - xj[d]);
       s = (d == D - 1) ? a : a + s;
     }
     return s;
@@ -299,7 +298,7 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T m = T(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const T a = __builtin_fabs(xi[d] - sx[d * cs + j]);
+      const T a = __builtin_fabs(xi[d] - xj[d]);
       m = (m < a) ? a : m;  // std::max(m, a)
     }
     return m;
@@ -307,7 +306,7 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T m = T(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const T a = rp.mparam[d] * __builtin_fabs(xi[d] - sx[d * cs + j]);
+      const T a = rp.mparam[d] * __builtin_fabs(xi[d] - xj[d]);
       m = (m < a) ? a : m;
     }
     return m;
@@ -315,12 +314,25 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
     T s = T(0);
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
-      const T dx = xi[d] - sx[d * cs + j];
-      const T sj = ssig[d * cs + j];
-      s = dx * dx / (si[d] * si[d] + sj * sj) + s;
+      const T dx = xi[d] - xj[d];
+      s = dx * dx / (si[d] * si[d] + sj[d] * sj[d]) + s;
     }
     return s;
   }
+}
+
+// The same for candidate j read from the sorted columns.
+template <int D, int M, class T>
+__device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
+                                          const T* __restrict__ sx, int64_t cs, int j,
+                                          const T* __restrict__ ssig, const RunParamsT<T>& rp) {
+  T xj[D], sj[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xj[d] = sx[d * cs + j];
+    sj[d] = (M == M_MAHALANOBIS) ? ssig[d * cs + j] : T(0);
+  }
+  return metric_pair<D, M>(xi, si, xj, sj, rp);
 }
 
 template <int M, class T>
@@ -523,8 +535,9 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
   if (t >= n_tiles) return;
   const int lane = lane_id();
   const int b = toff[t], e = toff[t + 1];
-  if (e - b <= kWave) {  // one item: the order is irrelevant
+  if (e - b <= kWave) {  // one item: the query order is irrelevant
     if (b + lane < e) {
+      qfine[b + lane] = b + lane;
       qperm[b + lane] = b + lane;
       qperm_nh[b + lane] = b + lane;
     }
@@ -762,6 +775,8 @@ __device__ __forceinline__ bool key_higher(uint64_t kj, uint32_t tj, uint64_t ki
 }
 __device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ uint64_t query_key(float rho, uint32_t tag) {
   return rho_tag_key(rho, tag, true);
 }
@@ -881,6 +896,321 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
 }
 
 // ---------------------------------------------------------------------------
+// (c') Nearest-higher over Morton-ordered candidate blocks. The accepted
+// candidate of the reference's scan is the one of smallest distance, equal
+// distances going to the larger (rho, tag); with unique tags (the default ones:
+// the input index) no two candidates tie on all three, so the result does not
+// depend on the order in which candidates are visited. Each tile's candidates
+// are laid out in Morton order in blocks of kNhBlock with the block's bounding
+// box and largest key; a wave skips a block when for each lane either no
+// candidate of the block is higher or the box lies beyond the lane's current
+// bound. A skipped block holds no candidate that the scan would accept (the box
+// bound never exceeds a candidate's computed distance value: rounding is
+// monotone), so the results are those of the plain scan.
+// ---------------------------------------------------------------------------
+
+// First layout block of tile t whose candidates begin at sorted position b:
+// tile starts rounded up to whole blocks, one spare block per tile, so that the
+// tiles' block ranges never overlap.
+__device__ __forceinline__ int64_t nh_first_block(int b, int t) {
+  return static_cast<int64_t>((b + kNhBlock - 1) / kNhBlock) + t;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  return static_cast<uint64_t>(__shfl_xor(static_cast<long long>(v), m, 64));
+}
+
+// One wave per tile: the tile's candidates in Morton order (qfine) into the
+// layout, padding to whole blocks, then each block's box and largest key.
+template <int D, class T>
+__global__ __launch_bounds__(kBlock) void k_nh_layout(
+    const int32_t* __restrict__ toff, int n_tiles, const int32_t* __restrict__ qfine,
+    const T* __restrict__ sx, int64_t cs, const uint64_t* __restrict__ skey, T* __restrict__ hx,
+    int64_t hs, uint64_t* __restrict__ hkey, int32_t* __restrict__ hidx, T* __restrict__ hbox,
+    uint64_t* __restrict__ hbkey) {
+  static_assert(kWave % kNhBlock == 0, "blocks must not straddle waves");
+  const int t = blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (t >= n_tiles) return;
+  const int lane = lane_id();
+  const int b = toff[t];
+  const int cnt = toff[t + 1] - b;
+  const int64_t blk0 = nh_first_block(b, t);
+  const int span = (cnt + kNhBlock - 1) / kNhBlock * kNhBlock;
+  for (int k0 = 0; k0 < span; k0 += kWave) {
+    const int k = k0 + lane;
+    const bool real = k < cnt;
+    T x[D];
+    uint64_t key = 0ull;
+    int s = -1;
+    if (real) {
+      s = qfine[b + k];
+      key = skey[s];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = real ? sx[d * cs + s] : T(0);
+    const int64_t p = blk0 * kNhBlock + k;
+    if (k < span) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) hx[d * hs + p] = x[d];
+      hkey[p] = key;
+      hidx[p] = s;
+    }
+    // Box and largest key of each block: reductions over its kNhBlock lanes.
+    T lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = real ? x[d] : std::numeric_limits<T>::infinity();
+      hi[d] = real ? x[d] : -std::numeric_limits<T>::infinity();
+    }
+    uint64_t mk = key;
+#pragma unroll
+    for (int off = 1; off < kNhBlock; off <<= 1) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        lo[d] = fmin_t(lo[d], __shfl_xor(lo[d], off, 64));
+        hi[d] = fmax_t(hi[d], __shfl_xor(hi[d], off, 64));
+      }
+      const uint64_t o = shfl_xor_u64(mk, off);
+      mk = o > mk ? o : mk;
+    }
+    if (k < span && (k % kNhBlock) == 0) {
+      const int64_t q = blk0 + k / kNhBlock;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        hbox[q * 2 * D + d] = lo[d];
+        hbox[q * 2 * D + D + d] = hi[d];
+      }
+      hbkey[q] = mk;
+    }
+  }
+}
+
+// A lower bound, over every candidate inside the box [bl, bh], of the value
+// metric_value computes (the same operations on the per-dimension gaps, which
+// never exceed the candidates' own differences after rounding). Weighted
+// metrics need weights >= 0 (checked on the host).
+template <int D, int M, class T>
+__device__ __forceinline__ T metric_lb(const T (&xi)[D], const T (&bl)[D], const T (&bh)[D],
+                                       const RunParamsT<T>& rp) {
+  T g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = fmax_t(fmax_t(bl[d] - xi[d], xi[d] - bh[d]), T(0));
+  if constexpr (M == M_EUCLID) {
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] * g[d] : g[d] * g[d] + s;
+    return s;
+  } else if constexpr (M == M_WEUCLID) {
+    T s = T(0);
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = rp.mparam[d] * g[d] * g[d] + s;
+    return s;
+  } else if constexpr (M == M_PERIODIC) {
+    // a = |dx| lies in [g, G]. With period P > 0, min(a, P - a) >= min(g, P - G)
+    // when P - G >= 0 (else 0 bounds its square). With P <= 0 (an unwrapped
+    // dimension), min(a, P - a) = P - a and |P - a| >= a >= g.
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const T P = rp.mparam[d];
+      const T G = fmax_t(xi[d] - bl[d], bh[d] - xi[d]);
+      const T pg = P - G;
+      const T m = (P <= T(0)) ? g[d] : ((pg >= T(0)) ? fmin_t(g[d], pg) : T(0));
+      s = (d == D - 1) ? m * m : m * m + s;
+    }
+    return s;
+  } else if constexpr (M == M_MANHATTAN) {
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] : g[d] + s;
+    return s;
+  } else if constexpr (M == M_CHEBYSHEV) {
+    T m = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) m = (m < g[d]) ? g[d] : m;
+    return m;
+  } else {  // M_WCHEBYSHEV
+    T m = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T a = rp.mparam[d] * g[d];
+      m = (m < a) ? a : m;
+    }
+    return m;
+  }
+}
+
+// Candidates per register block of the layout scan (a divisor of kNhBlock).
+template <int D, class T>
+constexpr int nh_sub_block() {
+  return sizeof(T) == 4 ? (D <= 3 ? 16 : 4) : (D <= 3 ? 8 : 4);
+}
+
+// Blocks whose boxes and keys are fetched together: as many as fit in about 128
+// bytes of scalar registers (a power of two, at most 4).
+template <int D, class T>
+constexpr int nh_meta_group() {
+  constexpr int bytes = 2 * D * static_cast<int>(sizeof(T)) + 8;
+  return 4 * bytes <= 128 ? 4 : (2 * bytes <= 128 ? 2 : 1);
+}
+
+// May a block whose largest key is bk hold a candidate higher than the query?
+// Float keys decide it exactly; double keys leave equal densities to the tags.
+__device__ __forceinline__ bool block_may_be_higher(uint64_t bk, uint64_t ki, float) {
+  return bk > ki;
+}
+__device__ __forceinline__ bool block_may_be_higher(uint64_t bk, uint64_t ki, double) {
+  return bk >= ki;
+}
+__device__ __forceinline__ bool key_maybe_higher(uint64_t kj, uint64_t ki, float) {
+  return kj > ki;
+}
+__device__ __forceinline__ bool key_maybe_higher(uint64_t kj, uint64_t ki, double) {
+  return (kj > ki) | ((kj == ki) & (kj > 0x8000000000000000ull));
+}
+
+template <int D, int M, class T>
+__global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
+    const int32_t* __restrict__ toff, const Item* __restrict__ items,
+    const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
+    const T* __restrict__ ssig, const T* __restrict__ su, const uint32_t* __restrict__ stag,
+    const T* __restrict__ srho, const uint64_t* __restrict__ skey,
+    const int32_t* __restrict__ qperm, const T* __restrict__ hx, int64_t hs,
+    const uint64_t* __restrict__ hkey, const int32_t* __restrict__ hidx,
+    const T* __restrict__ hbox, const uint64_t* __restrict__ hbkey, int32_t* __restrict__ snh,
+    T* __restrict__ sdelta) {
+  constexpr int V = nh_sub_block<D, T>();
+  constexpr int G = nh_meta_group<D, T>();
+  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
+                   threadIdx.x / kWave;
+  if (item >= *n_items) return;
+  const Item it = items[item];
+  const EventDesc ev = events[it.event];
+  const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
+  const int q = it.begin + lane_id();
+  const bool active = q < it.end;
+  const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
+
+  T xi[D], si[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xi[d] = active ? sx[d * cs + s] : T(0);
+    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
+  }
+  const T rho_i = active ? srho[s] : T(0);
+  const uint32_t tag_i = active ? stag[s] : 0u;
+  const uint64_t key_i = active ? query_key(rho_i, tag_i) : ~0ull;
+  const T ui = (active && rp.has_u) ? su[s] : T(1);
+  const T eff_min = rp.rhoc * ui;
+  const bool use_seed = rho_i >= eff_min;
+  const T eff = use_seed ? rp.seed_dc : rp.dm;
+  const T eff_thr =
+      MetricTraits<M>::kSqrt ? (use_seed ? rp.seed_sq_max : rp.dm_sq_max) : eff;
+
+  int lo[D], hi[D];
+  search_box<D>(g, xi, si, rp.dm, lo, hi);
+  if (!active) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = INT32_MAX;
+      hi[d] = INT32_MIN;
+    }
+  }
+  T delta = std::numeric_limits<T>::max();
+  int nh = -1;
+  T rho_nh = T(0);
+  uint32_t tag_nh = 0u;
+  T bound = eff_thr;  // prune_threshold(eff_thr, delta)
+
+  // The reference's acceptance test (ClusteringKernels.hpp:180-195).
+  auto consider = [&](T v, T rj, uint32_t tj, int j) {
+    const T dist = finish_distance<M>(v);
+    if (dist <= eff &&
+        ((dist < delta) ||
+         ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+      delta = dist;
+      nh = j;
+      rho_nh = rj;
+      tag_nh = tj;
+      bound = prune_threshold<M>(eff_thr, delta);
+    }
+  };
+
+  // One block's candidates, V at a time.
+  auto scan_block = [&](int64_t p0, bool in) {
+    for (int k0 = 0; k0 < kNhBlock; k0 += V) {
+      const int64_t p = p0 + k0;
+      T v[V];
+      T m = std::numeric_limits<T>::infinity();
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        T xj[D], sj[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          xj[d] = hx[d * hs + p + k];
+          sj[d] = T(0);
+        }
+        v[k] = metric_pair<D, M>(xi, si, xj, sj, rp);
+        const bool h = in & key_maybe_higher(hkey[p + k], key_i, T(0));
+        m = fmin_t(m, h ? v[k] : std::numeric_limits<T>::infinity());
+      }
+      if (ballot(m <= bound) != 0ull) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+          if (in & key_maybe_higher(hkey[p + k], key_i, T(0)) & (v[k] <= bound)) {
+            const int j = hidx[p + k];
+            const uint32_t tj = stag[j];
+            if (key_higher(hkey[p + k], tj, key_i, tag_i, T(0))) consider(v[k], srho[j], tj, j);
+          }
+        }
+      }
+    }
+  };
+
+  // in: the lane's search box holds tile t. The boxes and keys of G blocks arrive
+  // in one round trip and are tested with the bounds at that point.
+  auto scan_tile = [&](int t, bool in) {
+    const int b = toff[t];
+    const int nblk = (toff[t + 1] - b + kNhBlock - 1) / kNhBlock;
+    const int64_t blk0 = nh_first_block(b, t);
+    for (int g0 = 0; g0 < nblk; g0 += G) {
+      uint32_t wanted = 0u;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int64_t qb = blk0 + g0 + i;  // past the tile: padded, ignored below
+        T bl[D], bh[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          bl[d] = hbox[qb * 2 * D + d];
+          bh[d] = hbox[qb * 2 * D + D + d];
+        }
+        const bool need = in & (g0 + i < nblk) & block_may_be_higher(hbkey[qb], key_i, T(0)) &
+                          (metric_lb<D, M>(xi, bl, bh, rp) <= bound);
+        wanted |= (ballot(need) != 0ull) ? (1u << i) : 0u;
+      }
+      wanted = uniform(static_cast<int>(wanted));
+      while (wanted) {
+        const int i = __builtin_ctz(wanted);
+        wanted &= wanted - 1u;
+        scan_block((blk0 + g0 + i) * kNhBlock, in);
+      }
+    }
+  };
+
+  // The lanes' own tile first: the nearest candidates tighten the bounds early.
+  const int own = uniform(it.tile);  // wave-uniform: keeps the scan's loads scalar
+  scan_tile(own, active);
+  walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+    if (t != own) scan_tile(t, in);
+  });
+  if (active) {
+    snh[s] = nh;
+    sdelta[s] = delta;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // (d) Seed flags (ClusteringKernels.hpp:300-316) and the results in input order.
 // snh is rewritten in place as the follower link of the label stage:
 // -2 for a seed, -1 for a point without nearest-higher, else the sorted
@@ -988,7 +1318,43 @@ static void launch_nh_dm(const NeighbourLaunch<T>& L) {
 }
 
 template <int D, class T>
+static void launch_nh_layout_d(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
+  const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
+  hipLaunchKernelGGL((k_nh_layout<D, T>), dim3(grid), dim3(kBlock), 0, L.s, b.toff,
+                     b.n_tiles_total, b.qfine, b.sorted.x[0], L.cs, b.skey, b.hx, b.h_stride,
+                     b.hkey, b.hidx, b.hbox, b.hbkey);
+}
+
+template <int D, int M, class T>
+static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
+  hipLaunchKernelGGL((k_nh_blocks<D, M, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
+                     b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0],
+                     b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm_nh, b.hx, b.h_stride,
+                     b.hkey, b.hidx, b.hbox, b.hbkey, b.snh, b.sdelta);
+}
+
+// Nearest-higher over the block layout: the layout pass, then the scan.
+template <int D, class T>
+static void launch_nh_blocks_d(const NeighbourLaunch<T>& L) {
+  launch_nh_layout_d<D>(L);
+  switch (L.rp->metric) {
+    case M_EUCLID: launch_nh_blocks_dm<D, M_EUCLID>(L); break;
+    case M_WEUCLID: launch_nh_blocks_dm<D, M_WEUCLID>(L); break;
+    case M_PERIODIC: launch_nh_blocks_dm<D, M_PERIODIC>(L); break;
+    case M_MANHATTAN: launch_nh_blocks_dm<D, M_MANHATTAN>(L); break;
+    case M_CHEBYSHEV: launch_nh_blocks_dm<D, M_CHEBYSHEV>(L); break;
+    default: launch_nh_blocks_dm<D, M_WCHEBYSHEV>(L); break;
+  }
+}
+
+template <int D, class T>
 static void launch_neighbour_d(const NeighbourLaunch<T>& L, bool density) {
+  if (!density && L.rp->nh_blocks && L.rp->metric != M_MAHALANOBIS) {
+    launch_nh_blocks_d<D>(L);
+    return;
+  }
 #define CLUE_METRIC_CASE(MID)               \
   case MID:                                 \
     if (density)                            \

@@ -6,6 +6,9 @@
 namespace clue_dev {
 
 constexpr int kMaxDim = 10;
+// Candidates per block of the nearest-higher block layout (one bounding box and
+// one largest key per block).
+constexpr int kNhBlock = 16;
 
 // One event of a batch. All offsets are host-computed from the event sizes.
 struct EventDesc {
@@ -63,6 +66,7 @@ struct RunParamsT {
   T mparam[kMaxDim];
   int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none
   int32_t block_waves;     // neighbour stages: waves per workgroup (1, 2, 4, 8 or 16)
+  int32_t nh_blocks;       // nearest-higher over the Morton-ordered block layout
 };
 using RunParams = RunParamsT<float>;
 

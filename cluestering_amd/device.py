@@ -67,16 +67,19 @@ class Context:
 
     def set_tuning(self, group_queries: Optional[bool] = None,
                    xcd_chunk_log2: Optional[int] = None,
-                   block_waves: Optional[int] = None) -> None:
+                   block_waves: Optional[int] = None,
+                   nh_plain_scan: Optional[bool] = None) -> None:
         """Fix the launch tuning of later runs (speed only; results never change).
         With no arguments the built-in table chooses again."""
-        if group_queries is None and xcd_chunk_log2 is None and block_waves is None:
+        if (group_queries is None and xcd_chunk_log2 is None and block_waves is None
+                and nh_plain_scan is None):
             _capi.check(self._lib.clue_set_tuning(self.handle, None))
             return
         t = _capi.Tuning()
         t.group_queries = 1 if group_queries or group_queries is None else 0
         t.xcd_chunk_log2 = 4 if xcd_chunk_log2 is None else int(xcd_chunk_log2)
         t.block_waves = 4 if block_waves is None else int(block_waves)
+        t.nh_plain_scan = 1 if nh_plain_scan else 0
         _capi.check(self._lib.clue_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self, params: _capi.Params, sizes: Sequence[int]) -> dict:
@@ -87,7 +90,7 @@ class Context:
                                               sz.ctypes.data_as(C.POINTER(C.c_uint32)),
                                               len(sz), C.byref(t)))
         return {"group_queries": bool(t.group_queries), "xcd_chunk_log2": int(t.xcd_chunk_log2),
-                "block_waves": int(t.block_waves)}
+                "block_waves": int(t.block_waves), "nh_plain_scan": bool(t.nh_plain_scan)}
 
     def set_profiling(self, enabled: bool) -> None:
         _capi.check(self._lib.clue_set_profiling(self.handle, 1 if enabled else 0))

@@ -234,7 +234,10 @@ typedef struct {
   int32_t group_queries;  /* 1: order each tile's queries by box class and position */
   int32_t xcd_chunk_log2; /* deal 2^k consecutive work-item blocks per XCD turn; -1: none */
   int32_t block_waves;    /* waves per workgroup of the neighbour stages: 1..16, power of 2 (0: 4) */
-  int32_t reserved[5];
+  int32_t nh_plain_scan;  /* 1: nearest-higher scans whole tiles in sorted order; 0 (default): over
+                             Morton-ordered candidate blocks, skipping blocks that cannot hold the
+                             answer, whenever tags are the default ones and the metric allows it */
+  int32_t reserved[4];
 } clue_tuning;
 /* Fixes the tuning of the context's later runs and plans; NULL returns to the table. */
 clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning);

@@ -234,3 +234,62 @@ def test_determinism_and_graph_replay(gpu_ctx, oracle_mod):
         clus.make_clusters_batched(pts, w.sizes, case.metric, case.kernel)
         assert_same(pts.results(), ref, f"eager{rep}")
     plan.close()
+
+
+def _blobs(ndim, n, seed, spread=0.6, n_centres=8):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, 20, size=(n_centres, ndim))
+    lab = rng.integers(0, n_centres, n)
+    x = (centres[lab] + rng.normal(0, spread, size=(n, ndim))).T
+    return x, rng.uniform(0.5, 1.5, n)
+
+
+@pytest.mark.parametrize("metric", [None, K.EuclideanMetric([1.0, 2.5, 0.5]), K.ManhattanMetric(),
+                                    K.ChebyshevMetric(), K.ChebyshevMetric([2.0, 1.0, 0.5]),
+                                    K.PeriodicEuclideanMetric([0.0, 12.0, 0.0]),
+                                    K.PeriodicEuclideanMetric([7.0, 9.0, 25.0]),
+                                    K.EuclideanMetric([1.0, 0.0, 2.0])])
+def test_nh_block_layout_matches_plain_scan(gpu_ctx, oracle_mod, metric):
+    """The nearest-higher stage over Morton-ordered candidate blocks with box and
+    key culling gives the plain scan's bits, for every metric with a box bound
+    (a zero weight included), including flat-kernel densities full of exact ties
+    and a batch of events."""
+    from cluestering_amd import _capi
+    x, w = _blobs(3, 12000, 41)
+    wrapped = ()
+    if metric is not None and metric.tag == "periodic_euclidean":
+        wrapped = tuple(int(p > 0) for p in metric.params)
+        for d, p in enumerate(metric.params):
+            if p > 0:
+                x[d] = np.mod(x[d], p)
+    x = x.astype(np.float32)
+    cases = [Case(x, w, 1.1, 6.0, 1.6, kernel=K.GaussianKernel(1.0, 1.0), metric=metric,
+                  wrapped=wrapped, name=f"{metric}-gauss"),
+             Case(x, np.ones_like(w), 0.9, 4.0, 1.4, kernel=K.FlatKernel(1.0), metric=metric,
+                  wrapped=wrapped, name=f"{metric}-flat-ties"),
+             Case(x, w, 1.1, 6.0, 2.2, seed_dc=2.8, kernel=K.ExponentialKernel(0.5, 1.0),
+                  metric=metric, wrapped=wrapped, sizes=[3000, 64, 1, 8935], name=f"{metric}-batch")]
+    try:
+        for plain in (False, True):
+            gpu_ctx.set_tuning(group_queries=True, xcd_chunk_log2=4, block_waves=1,
+                               nh_plain_scan=plain)
+            for case in cases:
+                _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
+                case.name = f"{case.name.split(' plain=')[0]} plain={plain}"
+                _check(case, gpu_ctx, oracle_mod)
+    finally:
+        gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 5, 10])
+def test_nh_block_layout_dimensions(gpu_ctx, oracle_mod, ndim):
+    from cluestering_amd import _capi
+    x, w = _blobs(ndim, 6000, 60 + ndim, spread=0.5)
+    gpu_ctx.set_tuning(group_queries=True, xcd_chunk_log2=4, block_waves=1)
+    try:
+        for dtype in (np.float32, np.float64):
+            _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
+            _check(Case(x, w, 0.9 + 0.3 * ndim, 3.0, 1.2 + 0.3 * ndim, name=f"dim{ndim}-{dtype.__name__}",
+                        dtype=dtype), gpu_ctx, oracle_mod)
+    finally:
+        gpu_ctx.set_tuning()
