@@ -183,6 +183,13 @@ __device__ __forceinline__ uint64_t rho_tag_key(float r, uint32_t tag, bool quer
   return (static_cast<uint64_t>(o) << 32) | (r > 0.0f ? tag : 0u);
 }
 
+// rho of a float ordering key (rho_tag_key's high word inverted; -0 comes back as
+// +0, which compares equal).
+__device__ __forceinline__ float key_rho(uint64_t key) {
+  const uint32_t o = static_cast<uint32_t>(key >> 32);
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
 // static_cast<int>(float or double) with the x86 semantics the reference's
 // serial backend runs with (cvttss2si / cvttsd2si): NaN and out-of-range values
 // give INT_MIN.

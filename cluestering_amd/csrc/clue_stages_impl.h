@@ -1118,19 +1118,20 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
     }
   }
   T delta = std::numeric_limits<T>::max();
-  int nh = -1;
+  int64_t nh_slot = -1;  // layout slot of the nearest-higher found so far
   T rho_nh = T(0);
   uint32_t tag_nh = 0u;
   T bound = eff_thr;  // prune_threshold(eff_thr, delta)
 
   // The reference's acceptance test (ClusteringKernels.hpp:180-195).
-  auto consider = [&](T v, T rj, uint32_t tj, int j) {
+  auto consider = [&](T v, T rj, uint32_t tj, int64_t slot) {
     const T dist = finish_distance<M>(v);
     if (dist <= eff &&
         ((dist < delta) ||
-         ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+         ((dist == delta) && (nh_slot >= 0) &&
+          ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
       delta = dist;
-      nh = j;
+      nh_slot = slot;
       rho_nh = rj;
       tag_nh = tj;
       bound = prune_threshold<M>(eff_thr, delta);
@@ -1158,10 +1159,19 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
       if (ballot(m <= bound) != 0ull) {
 #pragma unroll
         for (int k = 0; k < V; ++k) {
-          if (in & key_maybe_higher(hkey[p + k], key_i, T(0)) & (v[k] <= bound)) {
-            const int j = hidx[p + k];
-            const uint32_t tj = stag[j];
-            if (key_higher(hkey[p + k], tj, key_i, tag_i, T(0))) consider(v[k], srho[j], tj, j);
+          const uint64_t kj = hkey[p + k];
+          if (in & key_maybe_higher(kj, key_i, T(0)) & (v[k] <= bound)) {
+            if constexpr (sizeof(T) == 4) {
+              // Float keys are exact and carry rho, and the tag when rho > 0: no
+              // dependent loads on this path.
+              const float rj = key_rho(kj);
+              const uint32_t tj = rj > 0.0f ? static_cast<uint32_t>(kj) : stag[hidx[p + k]];
+              consider(v[k], rj, tj, p + k);
+            } else {
+              const int j = hidx[p + k];
+              const uint32_t tj = stag[j];
+              if (key_higher(kj, tj, key_i, tag_i, T(0))) consider(v[k], srho[j], tj, p + k);
+            }
           }
         }
       }
@@ -1205,7 +1215,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
     if (t != own) scan_tile(t, in);
   });
   if (active) {
-    snh[s] = nh;
+    snh[s] = nh_slot >= 0 ? hidx[nh_slot] : -1;
     sdelta[s] = delta;
   }
 }
