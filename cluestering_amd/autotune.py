@@ -26,7 +26,7 @@ from .device import Clusterer, Context, Plan, PointsDevice
 
 # Candidate tunings: (group_queries, xcd_chunk_log2, block_waves).
 CANDIDATES: List[Tuple[int, int, int]] = [(g, x, b) for g in (0, 1) for x in (-1, 2, 4, 6, 8)
-                                          for b in (1, 2, 4)]
+                                          for b in (1, 2)]
 
 # Buckets by mean event size: (upper bound of the bucket, event size range of the
 # synthetic batch). The last bucket is one large event.

@@ -1254,6 +1254,21 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // where a chain stays among neighbouring tiles of one event. An event occupies
 // the same range of positions in input and sorted order.
 // ---------------------------------------------------------------------------
+// Pointer jumping on the follower links before the chase: each pass replaces a
+// link by its target's link when that target is itself a follower, so a pass
+// halves every chain (at least) and the chase's dependent loads shrink with it.
+// Every link still points to an ancestor on the same chain, whatever order the
+// threads run in, so the chase ends at the same seed or outlier.
+static __global__ __launch_bounds__(kBlock) void k_link_jump(int32_t* __restrict__ link, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int l = link[s];
+  if (l < 0) return;
+  const int l2 = link[l];
+  if (l2 >= 0) link[s] = l2;
+}
+constexpr int kLinkJumpPasses = 3;
+
 static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
                                                    const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
@@ -1495,6 +1510,8 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
                      out.delta, out.is_seed, bb.flag);
   scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
   mark(kStageLabels);
+  for (int pass = 0; pass < kLinkJumpPasses; ++pass)
+    hipLaunchKernelGGL(k_link_jump, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, bb.snh, n);
   hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, bb.events,
                      bb.sorted.perm, bb.snh, bb.rank, out.cluster, bb.seeds,
                      out.event_of_cluster, out.local_ids);
