@@ -985,16 +985,17 @@ __global__ __launch_bounds__(kBlock) void k_nh_layout(
   }
 }
 
-// A lower bound, over every candidate inside the box [bl, bh], of the value
-// metric_value computes (the same operations on the per-dimension gaps, which
-// never exceed the candidates' own differences after rounding). Weighted
-// metrics need weights >= 0 (checked on the host).
+// A lower bound, over every query point in [qlo, qhi] and every candidate inside
+// the box [bl, bh], of the value metric_value computes (the same operations on
+// the per-dimension gaps, which never exceed the pairs' own differences after
+// rounding). With qlo = qhi = x it bounds one query point. Weighted metrics
+// need weights >= 0 (checked on the host).
 template <int D, int M, class T>
-__device__ __forceinline__ T metric_lb(const T (&xi)[D], const T (&bl)[D], const T (&bh)[D],
-                                       const RunParamsT<T>& rp) {
+__device__ __forceinline__ T metric_lb(const T (&qlo)[D], const T (&qhi)[D], const T (&bl)[D],
+                                       const T (&bh)[D], const RunParamsT<T>& rp) {
   T g[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) g[d] = fmax_t(fmax_t(bl[d] - xi[d], xi[d] - bh[d]), T(0));
+  for (int d = 0; d < D; ++d) g[d] = fmax_t(fmax_t(bl[d] - qhi[d], qlo[d] - bh[d]), T(0));
   if constexpr (M == M_EUCLID) {
     T s;
 #pragma unroll
@@ -1013,7 +1014,7 @@ __device__ __forceinline__ T metric_lb(const T (&xi)[D], const T (&bl)[D], const
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) {
       const T P = rp.mparam[d];
-      const T G = fmax_t(xi[d] - bl[d], bh[d] - xi[d]);
+      const T G = fmax_t(qhi[d] - bl[d], bh[d] - qlo[d]);
       const T pg = P - G;
       const T m = (P <= T(0)) ? g[d] : ((pg >= T(0)) ? fmin_t(g[d], pg) : T(0));
       s = (d == D - 1) ? m * m : m * m + s;
@@ -1040,18 +1041,49 @@ __device__ __forceinline__ T metric_lb(const T (&xi)[D], const T (&bl)[D], const
   }
 }
 
+// Wave-wide minimum and maximum of floating-point values and of unsigned keys.
+template <class T>
+__device__ __forceinline__ T wave_min_f(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin_t(v, __shfl_xor(v, off, 64));
+  return v;
+}
+template <class T>
+__device__ __forceinline__ T wave_max_f(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax_t(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = shfl_xor_u64(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+// Lane i's value in every lane (i wave-uniform).
+__device__ __forceinline__ float read_lane(float v, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
+}
+__device__ __forceinline__ double read_lane(double v, int i) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), i);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), i);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<uint32_t>(lo)));
+}
+__device__ __forceinline__ uint64_t read_lane(uint64_t v, int i) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), i));
+  const uint32_t hi =
+      static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), i));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Candidates per register block of the layout scan (a divisor of kNhBlock).
 template <int D, class T>
 constexpr int nh_sub_block() {
   return sizeof(T) == 4 ? (D <= 3 ? 16 : 4) : (D <= 3 ? 8 : 4);
-}
-
-// Blocks whose boxes and keys are fetched together: as many as fit in about 128
-// bytes of scalar registers (a power of two, at most 4).
-template <int D, class T>
-constexpr int nh_meta_group() {
-  constexpr int bytes = 2 * D * static_cast<int>(sizeof(T)) + 8;
-  return 4 * bytes <= 128 ? 4 : (2 * bytes <= 128 ? 2 : 1);
 }
 
 // May a block whose largest key is bk hold a candidate higher than the query?
@@ -1081,7 +1113,6 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
     const T* __restrict__ hbox, const uint64_t* __restrict__ hbkey, int32_t* __restrict__ snh,
     T* __restrict__ sdelta) {
   constexpr int V = nh_sub_block<D, T>();
-  constexpr int G = nh_meta_group<D, T>();
   const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
                    threadIdx.x / kWave;
   if (item >= *n_items) return;
@@ -1178,32 +1209,47 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
     }
   };
 
-  // in: the lane's search box holds tile t. The boxes and keys of G blocks arrive
-  // in one round trip and are tested with the bounds at that point.
+  // in: the lane's search box holds tile t. The boxes and keys of up to 64 blocks
+  // arrive in one vector load (one block per lane); a wave-wide test against the
+  // box, smallest key and largest bound of the lanes that hold the tile drops
+  // most blocks at once, and each block left is tested per lane before its scan.
   auto scan_tile = [&](int t, bool in) {
     const int b = toff[t];
     const int nblk = (toff[t + 1] - b + kNhBlock - 1) / kNhBlock;
     const int64_t blk0 = nh_first_block(b, t);
-    for (int g0 = 0; g0 < nblk; g0 += G) {
-      uint32_t wanted = 0u;
+    T qlo[D], qhi[D];
 #pragma unroll
-      for (int i = 0; i < G; ++i) {
-        const int64_t qb = blk0 + g0 + i;  // past the tile: padded, ignored below
-        T bl[D], bh[D];
+    for (int d = 0; d < D; ++d) {
+      qlo[d] = wave_min_f(in ? xi[d] : std::numeric_limits<T>::infinity());
+      qhi[d] = wave_max_f(in ? xi[d] : -std::numeric_limits<T>::infinity());
+    }
+    const uint64_t kmin = wave_min_u64(in ? key_i : ~0ull);
+    for (int c0 = 0; c0 < nblk; c0 += kWave) {
+      const T bmax = wave_max_f(in ? bound : -std::numeric_limits<T>::infinity());
+      const int l = lane_id();
+      const bool valid = c0 + l < nblk;
+      const int64_t qb = blk0 + c0 + (valid ? l : 0);
+      T bl[D], bh[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        bl[d] = hbox[qb * 2 * D + d];
+        bh[d] = hbox[qb * 2 * D + D + d];
+      }
+      const uint64_t bk = hbkey[qb];
+      uint64_t cand = ballot(valid & block_may_be_higher(bk, kmin, T(0)) &
+                             (metric_lb<D, M>(qlo, qhi, bl, bh, rp) <= bmax));
+      while (cand) {
+        const int i = __builtin_ctzll(cand);
+        cand &= cand - 1ull;
+        T sl[D], sh[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          bl[d] = hbox[qb * 2 * D + d];
-          bh[d] = hbox[qb * 2 * D + D + d];
+          sl[d] = read_lane(bl[d], i);
+          sh[d] = read_lane(bh[d], i);
         }
-        const bool need = in & (g0 + i < nblk) & block_may_be_higher(hbkey[qb], key_i, T(0)) &
-                          (metric_lb<D, M>(xi, bl, bh, rp) <= bound);
-        wanted |= (ballot(need) != 0ull) ? (1u << i) : 0u;
-      }
-      wanted = uniform(static_cast<int>(wanted));
-      while (wanted) {
-        const int i = __builtin_ctz(wanted);
-        wanted &= wanted - 1u;
-        scan_block((blk0 + g0 + i) * kNhBlock, in);
+        const bool need = in & block_may_be_higher(read_lane(bk, i), key_i, T(0)) &
+                          (metric_lb<D, M>(xi, xi, sl, sh, rp) <= bound);
+        if (ballot(need) != 0ull) scan_block((blk0 + c0 + i) * kNhBlock, in);
       }
     }
   };
