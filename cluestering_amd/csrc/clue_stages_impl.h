@@ -590,9 +590,11 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
 // Kernel values (ConvolutionalKernel.hpp:21-57) for a candidate within the radius.
 // K_GAUSS2 is the gaussian whose 2*sigma*sigma is a power of two: there division
 // by it and multiplication by its reciprocal are the same correctly rounded
-// operation on the same exact value, so the cheaper one is used.
+// operation on the same exact value, so the cheaper one is used. The _UNIT forms
+// are those of amplitude 1, whose product A * e is e itself (denormals are kept,
+// so even a subnormal e is unchanged).
 // ---------------------------------------------------------------------------
-enum { K_GAUSS2 = 3 };
+enum { K_GAUSS2 = 3, K_EXP_UNIT = 4, K_GAUSS2_UNIT = 5 };
 
 // exp of an argument the host has shown to lie in [LO, HI] for every candidate
 // within the radius (rp.exp_in_range) skips the range handling; both forms give
@@ -608,14 +610,16 @@ template <int M, int K, class T>
 __device__ __forceinline__ T kernel_value(T v, const RunParamsT<T>& rp) {
   if constexpr (K == K_FLAT) {
     return rp.kp0;
-  } else if constexpr (K == K_EXP) {
-    return rp.kp1 * kernel_exp(-rp.kp0 * finish_distance<M>(v), rp);
+  } else if constexpr (K == K_EXP || K == K_EXP_UNIT) {
+    const T e = kernel_exp(-rp.kp0 * finish_distance<M>(v), rp);
+    return K == K_EXP_UNIT ? e : rp.kp1 * e;
   } else if constexpr (K == K_GAUSS) {
     const T dd = finish_distance<M>(v);
     return rp.kp1 * kernel_exp(-dd * dd / rp.gauss_den, rp);
   } else {
     const T dd = finish_distance<M>(v);
-    return rp.kp1 * kernel_exp((-dd * dd) * rp.gauss_inv_den, rp);
+    const T e = kernel_exp((-dd * dd) * rp.gauss_inv_den, rp);
+    return K == K_GAUSS2_UNIT ? e : rp.kp1 * e;
   }
 }
 
@@ -1369,9 +1373,16 @@ template <int D, int M, class T>
 static void launch_density_dm(const NeighbourLaunch<T>& L) {
   switch (L.rp->kernel) {
     case K_FLAT: launch_density_dmk<D, M, K_FLAT>(L); break;
-    case K_EXP: launch_density_dmk<D, M, K_EXP>(L); break;
+    case K_EXP:
+      if (L.rp->kp1 == T(1))
+        launch_density_dmk<D, M, K_EXP_UNIT>(L);
+      else
+        launch_density_dmk<D, M, K_EXP>(L);
+      break;
     default:
-      if (L.rp->gauss_pow2)
+      if (L.rp->gauss_pow2 && L.rp->kp1 == T(1))
+        launch_density_dmk<D, M, K_GAUSS2_UNIT>(L);
+      else if (L.rp->gauss_pow2)
         launch_density_dmk<D, M, K_GAUSS2>(L);
       else
         launch_density_dmk<D, M, K_GAUSS>(L);
