@@ -1085,9 +1085,9 @@ __device__ __forceinline__ uint64_t read_lane(uint64_t v, int i) {
 }
 
 // Candidates per register block of the layout scan (a divisor of kNhBlock).
-template <int D, class T>
+template <int D>
 constexpr int nh_sub_block() {
-  return sizeof(T) == 4 ? (D <= 3 ? 16 : 4) : (D <= 3 ? 8 : 4);
+  return D <= 3 ? 8 : 4;
 }
 
 // May a block whose largest key is bk hold a candidate higher than the query?
@@ -1105,8 +1105,13 @@ __device__ __forceinline__ bool key_maybe_higher(uint64_t kj, uint64_t ki, doubl
   return (kj > ki) | ((kj == ki) & (kj > 0x8000000000000000ull));
 }
 
+// Eight waves per SIMD: the scan waits on chains of scalar loads, so resident
+// waves matter more than registers (measured on MI355X on the 4096-event batch:
+// 12.3 ms with the compiler's own choice, 11.0 ms with this bound, 10.2 ms with
+// it and register blocks of 8 candidates instead of 16).
 template <int D, int M, class T>
-__global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
+__global__ __launch_bounds__(kMaxNeighbourBlock) __attribute__((amdgpu_waves_per_eu(8))) void
+k_nh_blocks(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
@@ -1116,7 +1121,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nh_blocks(
     const uint64_t* __restrict__ hkey, const int32_t* __restrict__ hidx,
     const T* __restrict__ hbox, const uint64_t* __restrict__ hbkey, int32_t* __restrict__ snh,
     T* __restrict__ sdelta) {
-  constexpr int V = nh_sub_block<D, T>();
+  constexpr int V = nh_sub_block<D>();
   const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
                    threadIdx.x / kWave;
   if (item >= *n_items) return;
