@@ -4,9 +4,12 @@
 //   (b) local density and (c) nearest-higher: one wave per work item (up to 64
 //       points of one tile, one lane each) walks the union of the lanes' search
 //       boxes in the reference's lexicographic tile order; candidates are
-//       wave-uniform and come through the scalar cache;
+//       wave-uniform and come through the scalar cache. With the default tags,
+//       (c) instead scans each tile's candidates in Morton-ordered blocks and
+//       skips the blocks that cannot hold the answer (its result does not
+//       depend on the visiting order);
 //   (d) seeds, numbered by an exclusive scan in input order;
-//   (e) follower labels.
+//   (e) follower labels: pointer jumping, then a chase to the seed.
 //
 // Semantics follow the reference's serial backend:
 //   geometry  core/detail/SetupTiles.hpp:23-57, core/detail/ComputeTiles.hpp:17-37
