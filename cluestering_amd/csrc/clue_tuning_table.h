@@ -18,7 +18,7 @@ inline constexpr TuningEntry kTuningTable[] = {
     {2, 2048, 0, 8, 1},
     {2, 16384, 1, 4, 1},
     {2, 131072, 1, 6, 1},
-    {2, INT64_MAX, 1, -1, 1},
+    {2, INT64_MAX, 1, 6, 1},
     {3, 2048, 1, 6, 1},
     {3, 16384, 1, 6, 1},
     {3, 131072, 1, 8, 1},
