@@ -273,7 +273,7 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
       {S_NITEMS, sizeof(int32_t) * 4},
       {S_ITEMS, sizeof(Item) * bp.max_items},
       {S_QPERM, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
-      {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc) ? sizeof(int32_t) * n : 0},
+      {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc && !nhb) ? sizeof(int32_t) * n : 0},
       {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
       {S_QKEY, bp.tuning.group_queries ? sizeof(uint32_t) * n : 0},
       {S_HX, sizeof(T) * h_slots * D},
@@ -330,8 +330,10 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
   b.n_items = ws.buf[S_NITEMS].as<int32_t>();
   b.items = ws.buf[S_ITEMS].as<Item>();
   b.qperm = bp.tuning.group_queries ? ws.buf[S_QPERM].as<int32_t>() : nullptr;
-  b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc) ? ws.buf[S_QPERM_NH].as<int32_t>()
-                                                          : b.qperm;
+  // The block scan orders its queries by qfine: no separate order for dm.
+  b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc && !nhb)
+                   ? ws.buf[S_QPERM_NH].as<int32_t>()
+                   : b.qperm;
   b.qfine = bp.tuning.group_queries ? ws.buf[S_QFINE].as<int32_t>() : nullptr;
   b.qkey = bp.tuning.group_queries ? ws.buf[S_QKEY].as<uint32_t>() : nullptr;
   b.srho = ws.buf[S_SRHO].as<T>();

@@ -1421,11 +1421,13 @@ static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
   hipLaunchKernelGGL((k_nh_blocks<D, M, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
                      b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0],
-                     b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qperm_nh, b.hx, b.h_stride,
+                     b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qfine, b.hx, b.h_stride,
                      b.hkey, b.hidx, b.hbox, b.hbkey, b.snh, b.sdelta);
 }
 
-// Nearest-higher over the block layout: the layout pass, then the scan.
+// Nearest-higher over the block layout: the layout pass, then the scan. Its
+// waves take each tile's points in Morton order (qfine): the scan visits only the
+// blocks its lanes want, so grouping lanes by search-box class buys nothing here.
 template <int D, class T>
 static void launch_nh_blocks_d(const NeighbourLaunch<T>& L) {
   launch_nh_layout_d<D>(L);
