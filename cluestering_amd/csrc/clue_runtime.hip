@@ -279,10 +279,8 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
       {S_HX, sizeof(T) * h_slots * D},
       {S_HKEY, sizeof(uint64_t) * h_slots},
       {S_HIDX, sizeof(int32_t) * h_slots},
-      // Block boxes and keys are read in groups of up to 4 from a tile's first
-      // block: 4 spare entries keep the last group in bounds.
-      {S_HBOX, nhb ? sizeof(T) * (h_blocks + 4) * 2 * D : 0},
-      {S_HBKEY, nhb ? sizeof(uint64_t) * (h_blocks + 4) : 0},
+      {S_HBOX, sizeof(T) * h_blocks * 2 * D},
+      {S_HBKEY, sizeof(uint64_t) * h_blocks},
       {S_SRHO, sizeof(T) * n},
       {S_SKEY, sizeof(uint64_t) * n},
       {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(T) * n : 0},
