@@ -1267,8 +1267,22 @@ k_nh_blocks(
   };
 
   // The lanes' own tile first: the nearest candidates tighten the bounds early.
+  // A lane scans it only if its box holds it: on a wrapped dimension a box wider
+  // than half the data's extent can leave it out (the walk below reaches the
+  // own tile at bin b or, wrapped, at b + nper).
   const int own = uniform(it.tile);  // wave-uniform: keeps the scan's loads scalar
-  scan_tile(own, active);
+  {
+    bool in_own = active;
+    int r = own - ev.t_base;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const int b = r % ev.nper;
+      r /= ev.nper;
+      in_own = in_own && (((lo[d] <= b) && (b <= hi[d])) ||
+                          (rp.wrapped[d] && (lo[d] <= b + ev.nper) && (b + ev.nper <= hi[d])));
+    }
+    if (ballot(in_own) != 0ull) scan_tile(own, in_own);
+  }
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
     if (t != own) scan_tile(t, in);
   });

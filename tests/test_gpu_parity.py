@@ -268,7 +268,10 @@ def test_nh_block_layout_matches_plain_scan(gpu_ctx, oracle_mod, metric):
              Case(x, np.ones_like(w), 0.9, 4.0, 1.4, kernel=K.FlatKernel(1.0), metric=metric,
                   wrapped=wrapped, name=f"{metric}-flat-ties"),
              Case(x, w, 1.1, 6.0, 2.2, seed_dc=2.8, kernel=K.ExponentialKernel(0.5, 1.0),
-                  metric=metric, wrapped=wrapped, sizes=[3000, 64, 1, 8935], name=f"{metric}-batch")]
+                  metric=metric, wrapped=wrapped, sizes=[3000, 64, 1, 8935], name=f"{metric}-batch"),
+             # A search box wider than the grid: with wrapping, tiles recur in the walk.
+             Case(x[:, :3000], w[:3000], 0.8, 2.0, 4.5, kernel=K.FlatKernel(0.5), metric=metric,
+                  wrapped=wrapped, name=f"{metric}-wide-box")]
     try:
         for plain in (False, True):
             gpu_ctx.set_tuning(group_queries=True, xcd_chunk_log2=4, block_waves=1,
