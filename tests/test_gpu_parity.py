@@ -296,3 +296,24 @@ def test_nh_block_layout_dimensions(gpu_ctx, oracle_mod, ndim):
                         dtype=dtype), gpu_ctx, oracle_mod)
     finally:
         gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_duplicates_zero_and_negative_weights(gpu_ctx, oracle_mod, plain):
+    """Exact duplicate points (zero distances, equal densities broken by tag) and
+    weights that are zero or negative (densities <= 0, whose keys carry no tag),
+    through both nearest-higher scans, bit for bit against the oracle."""
+    from cluestering_amd import _capi
+    rng = np.random.default_rng(77)
+    base = rng.uniform(0, 12, size=(2, 1500)).astype(np.float32)
+    x = np.concatenate([base, base[:, :700], base[:, :300]], axis=1)  # duplicates
+    n = x.shape[1]
+    w = rng.choice(np.array([1.0, 0.0, -0.5, 2.0], np.float32), size=n, p=[0.55, 0.15, 0.15, 0.15])
+    gpu_ctx.set_tuning(group_queries=True, xcd_chunk_log2=4, block_waves=1, nh_plain_scan=plain)
+    try:
+        for kernel in (K.FlatKernel(0.5), K.GaussianKernel(1.0, 1.0)):
+            _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
+            _check(Case(x, w, 0.9, 0.5, 1.3, kernel=kernel, name=f"dups-{kernel} plain={plain}"),
+                   gpu_ctx, oracle_mod)
+    finally:
+        gpu_ctx.set_tuning()
