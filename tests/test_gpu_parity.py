@@ -298,8 +298,9 @@ def test_nh_block_layout_dimensions(gpu_ctx, oracle_mod, ndim):
         gpu_ctx.set_tuning()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("plain", [False, True])
-def test_duplicates_zero_and_negative_weights(gpu_ctx, oracle_mod, plain):
+def test_duplicates_zero_and_negative_weights(gpu_ctx, oracle_mod, plain, dtype):
     """Exact duplicate points (zero distances, equal densities broken by tag) and
     weights that are zero or negative (densities <= 0, whose keys carry no tag),
     through both nearest-higher scans, bit for bit against the oracle."""
@@ -313,7 +314,8 @@ def test_duplicates_zero_and_negative_weights(gpu_ctx, oracle_mod, plain):
     try:
         for kernel in (K.FlatKernel(0.5), K.GaussianKernel(1.0, 1.0)):
             _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
-            _check(Case(x, w, 0.9, 0.5, 1.3, kernel=kernel, name=f"dups-{kernel} plain={plain}"),
+            _check(Case(x, w, 0.9, 0.5, 1.3, kernel=kernel, dtype=dtype,
+                        name=f"dups-{kernel} plain={plain} {np.dtype(dtype).name}"),
                    gpu_ctx, oracle_mod)
     finally:
         gpu_ctx.set_tuning()
