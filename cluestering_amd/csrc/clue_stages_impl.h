@@ -1108,13 +1108,15 @@ __device__ __forceinline__ bool key_maybe_higher(uint64_t kj, uint64_t ki, doubl
   return (kj > ki) | ((kj == ki) & (kj > 0x8000000000000000ull));
 }
 
-// Eight waves per SIMD: the scan waits on chains of scalar loads, so resident
-// waves matter more than registers (measured on MI355X on the 4096-event batch:
-// 12.3 ms with the compiler's own choice, 11.0 ms with this bound, 10.2 ms with
-// it and register blocks of 8 candidates instead of 16).
+// Eight waves per SIMD in 1-D and 2-D: the scan waits on chains of scalar loads,
+// so resident waves matter more than registers (measured on MI355X on the
+// 4096-event 2-D batch: 12.3 ms with the compiler's own choice, 11.0 ms with
+// this bound, 10.2 ms with it and register blocks of 8 candidates instead of
+// 16). From 3-D on the bound would spill to scratch; there the compiler chooses
+// (equal time on the 2^20-point 3-D event).
 template <int D, int M, class T>
-__global__ __launch_bounds__(kMaxNeighbourBlock) __attribute__((amdgpu_waves_per_eu(8))) void
-k_nh_blocks(
+__global__ __launch_bounds__(kMaxNeighbourBlock)
+__attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
