@@ -319,3 +319,18 @@ def test_duplicates_zero_and_negative_weights(gpu_ctx, oracle_mod, plain, dtype)
                    gpu_ctx, oracle_mod)
     finally:
         gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_large_tiles(gpu_ctx, oracle_mod, plain):
+    """Tiles of thousands of points (points_per_tile = 3000): the block scan reads a
+    tile's block boxes in several rounds of 64."""
+    from cluestering_amd import _capi
+    x, w = _blobs(2, 20000, 88, spread=1.5, n_centres=4)
+    gpu_ctx.set_tuning(group_queries=True, xcd_chunk_log2=4, block_waves=1, nh_plain_scan=plain)
+    try:
+        _capi.check(_capi.load().clue_debug_poison(gpu_ctx.handle))
+        _check(Case(x, w, 0.7, 4.0, 1.1, kernel=K.GaussianKernel(1.0, 1.0), points_per_tile=3000,
+                    name=f"large-tiles plain={plain}"), gpu_ctx, oracle_mod)
+    finally:
+        gpu_ctx.set_tuning()
