@@ -23,6 +23,26 @@ PeriodicEuclideanMetric = clue_kernels.PeriodicEuclideanMetric
 
 backends = ["gpu hip"]
 
+_MAX_DIM = 10
+_DIM_ERROR = "Inadequate data. The supported dimensions are between 1 and 10."
+# choice -> (constructor, number of parameters, name in messages)
+_KERNELS = {
+    "flat": (clue_kernels.FlatKernel, 1, "flat"),
+    "exp": (clue_kernels.ExponentialKernel, 2, "exponential"),
+    "gaus": (clue_kernels.GaussianKernel, 2, "gaussian"),
+}
+_PLAIN_METRICS = {
+    "euclidean": clue_kernels.EuclideanMetric,
+    "manhattan": clue_kernels.ManhattanMetric,
+    "chebyshev": clue_kernels.ChebyshevMetric,
+}
+# choice -> (constructor taking the per-dimension list, what the list holds)
+_PARAM_METRICS = {
+    "weighted_euclidean": (clue_kernels.EuclideanMetric, "weights"),
+    "weighted_chebyshev": (clue_kernels.ChebyshevMetric, "weights"),
+    "periodic_euclidean": (clue_kernels.PeriodicEuclideanMetric, "periods"),
+}
+
 
 def is_hip_available() -> bool:
     from .device import device_count
@@ -85,56 +105,60 @@ class clusterer:
         self._seeding_distance = density_radius if seeding_distance is None else seeding_distance
 
     # -- input (CLUEstering.py:291-404) -------------------------------------------------
+    # Every input form ends as one (Ndim+1) x N C-contiguous array: the coordinate
+    # rows, then the weight row. Files, dicts and DataFrames are read as float32 by
+    # default; row lists and arrays keep the floating type of their first value.
+    def _store_soa(self, rows, dtype) -> None:
+        n_points = len(rows[-1])
+        soa = np.empty((len(rows), n_points), dtype=dtype)
+        for k, row in enumerate(rows):
+            row = np.asarray(row)
+            if row.shape != (n_points,):
+                raise ValueError(f"row {k} has {row.size} values where {n_points} are expected")
+            soa[k] = row
+        self.clust_data = ClusteringDataSoA(soa, np.zeros(n_points, dtype=np.int32),
+                                            len(rows) - 1, n_points)
+
     def _read_array(self, input_data) -> None:
-        if len(input_data) < 2 or len(input_data) > 11:
-            raise ValueError("Inadequate data. The supported dimensions are between" +
-                             "1 and 10.")
-        npoints = len(input_data[-1])
-        ndim = len(input_data[:-1])
-        dtype = np.asarray(input_data[0][0]).dtype
-        if not np.issubdtype(dtype, np.floating):
-            dtype = np.float64
-        coords = np.vstack([np.asarray(input_data[:-1], dtype=dtype),
-                            np.asarray(input_data[-1], dtype=dtype)])
-        coords = np.ascontiguousarray(coords, dtype=dtype)
-        self.clust_data = ClusteringDataSoA(coords, np.zeros(npoints, dtype=np.int32), ndim,
-                                            npoints)
+        if not 2 <= len(input_data) <= _MAX_DIM + 1:
+            raise ValueError(_DIM_ERROR)
+        first = np.asarray(input_data[0]).reshape(-1)[:1]
+        dtype = first.dtype if first.size and np.issubdtype(first.dtype, np.floating) \
+            else np.dtype(np.float64)
+        self._store_soa(list(input_data), dtype)
 
     def _read_string(self, input_data: str, dtype=np.float32):
-        import pandas as pd
         if not input_data.endswith(".csv"):
             raise ValueError("Wrong type of file. The file is not a csv file.")
+        import pandas as pd
         return pd.read_csv(input_data, dtype=dtype)
 
     def _read_dict_df(self, input_data, dtype=np.float32):
         import pandas as pd
-        return pd.DataFrame(input_data, copy=False, dtype=dtype)
+        frame = input_data if isinstance(input_data, pd.DataFrame) else pd.DataFrame(input_data)
+        return frame.astype(dtype, copy=False)
 
     def _handle_dataframe(self, df_) -> None:
-        if len(df_.columns) < 2:
+        n_cols = df_.shape[1]
+        if n_cols < 2:
             raise ValueError("Inadequate data. The data must contain at least one coordinate "
                              "and the weight.")
-        if len(df_.columns) > 11:
+        if n_cols > _MAX_DIM + 1:
             raise ValueError("Inadequate data. The maximum number of dimensions supported is 10.")
-        ndim = len(df_.columns) - 1
-        npoints = len(df_.index)
-        coords = df_.iloc[:, 0:-1].to_numpy()
-        coords = np.vstack([coords.T, df_.iloc[:, -1]]).astype(df_.dtypes.iloc[0])
-        coords = np.ascontiguousarray(coords, dtype=df_.dtypes.iloc[0])
-        self.clust_data = ClusteringDataSoA(coords, np.zeros(npoints, dtype=np.int32), ndim,
-                                            npoints)
+        rows = [df_.iloc[:, k].to_numpy() for k in range(n_cols)]
+        self._store_soa(rows, rows[0].dtype)
 
     def read_data(self, input_data, wrapped_coords=None) -> None:
         """Reads a DataFrame, a CSV path, a dict, or a list/array of coordinate rows
         followed by the weight row."""
         import pandas as pd
-        if isinstance(input_data, (list, np.ndarray)):
-            self._read_array(input_data)
         if isinstance(input_data, str):
             self._handle_dataframe(self._read_string(input_data))
-        if isinstance(input_data, (dict, pd.DataFrame)):
+        elif isinstance(input_data, (dict, pd.DataFrame)):
             self._handle_dataframe(self._read_dict_df(input_data))
-        if self.clust_data is None:
+        elif isinstance(input_data, (list, np.ndarray)):
+            self._read_array(input_data)
+        else:
             raise ValueError("unsupported input data type")
         self.wrapped = list(wrapped_coords) if wrapped_coords is not None else \
             [0] * self.clust_data.n_dim
@@ -144,49 +168,26 @@ class clusterer:
 
     # -- kernel and metric (CLUEstering.py:418-496) -------------------------------------
     def choose_kernel(self, choice: str, parameters=None) -> None:
-        if choice == "flat":
-            if len(parameters) != 1:
-                raise ValueError("Wrong number of parameters. The flat kernel requires 1 parameter.")
-            self._kernel = clue_kernels.FlatKernel(parameters[0])
-        elif choice == "exp":
-            if len(parameters) != 2:
-                raise ValueError("Wrong number of parameters. The exponential kernel requires 2 "
-                                 "parameters.")
-            self._kernel = clue_kernels.ExponentialKernel(parameters[0], parameters[1])
-        elif choice == "gaus":
-            if len(parameters) != 2:
-                raise ValueError("Wrong number of parameters. The gaussian kernel requires 2 "
-                                 "parameters.")
-            self._kernel = clue_kernels.GaussianKernel(parameters[0], parameters[1])
-        else:
+        if choice not in _KERNELS:
             raise ValueError("Invalid kernel. Allowed choices are: flat, exp, gaus.")
+        make, n_params, name = _KERNELS[choice]
+        if parameters is None or len(parameters) != n_params:
+            plural = "parameter" if n_params == 1 else "parameters"
+            raise ValueError(f"Wrong number of parameters. The {name} kernel requires "
+                             f"{n_params} {plural}.")
+        self._kernel = make(*parameters)
 
     def choose_metric(self, choice: str, parameters=None) -> None:
-        if choice == "euclidean":
-            self._metric = clue_kernels.EuclideanMetric()
-        elif choice == "manhattan":
-            self._metric = clue_kernels.ManhattanMetric()
-        elif choice == "chebyshev":
-            self._metric = clue_kernels.ChebyshevMetric()
-        elif choice == "weighted_euclidean":
+        if choice in _PLAIN_METRICS:
+            self._metric = _PLAIN_METRICS[choice]()
+        elif choice in _PARAM_METRICS:
+            make, what = _PARAM_METRICS[choice]
             if not parameters:
-                raise ValueError("weighted_euclidean requires a 'parameters' list of "
-                                 "per-dimension weights.")
-            self._metric = clue_kernels.EuclideanMetric(list(parameters))
-        elif choice == "weighted_chebyshev":
-            if not parameters:
-                raise ValueError("weighted_chebyshev requires a 'parameters' list of "
-                                 "per-dimension weights.")
-            self._metric = clue_kernels.ChebyshevMetric(list(parameters))
-        elif choice == "periodic_euclidean":
-            if not parameters:
-                raise ValueError("periodic_euclidean requires a 'parameters' list of "
-                                 "per-dimension periods.")
-            self._metric = clue_kernels.PeriodicEuclideanMetric(list(parameters))
+                raise ValueError(f"{choice} requires a 'parameters' list of per-dimension {what}.")
+            self._metric = make(list(parameters))
         else:
-            raise ValueError("Invalid metric. Allowed choices are: euclidean, manhattan, "
-                             "chebyshev, weighted_euclidean, weighted_chebyshev, "
-                             "periodic_euclidean.")
+            raise ValueError("Invalid metric. Allowed choices are: " +
+                             ", ".join(list(_PLAIN_METRICS) + list(_PARAM_METRICS)) + ".")
 
     # -- accessors -------------------------------------------------------------------------
     @property
@@ -212,9 +213,8 @@ class clusterer:
             raise ValueError(f"Invalid backend. This package provides: {', '.join(backends)}.")
 
     def _partial_dimension_dataset(self, dimensions) -> np.ndarray:
-        coords = [np.copy(self.clust_data.coords[dim]) for dim in dimensions]
-        coords.append(np.copy(self.clust_data.coords[-1]))
-        return np.ascontiguousarray(coords, dtype=self.clust_data.coords.dtype)
+        # The chosen coordinate rows and the weight row, as a new contiguous SoA.
+        return np.ascontiguousarray(self.clust_data.coords[list(dimensions) + [-1]])
 
     # -- run (CLUEstering.py:609-690) ------------------------------------------------------
     def run_clue(self, backend: str = "gpu hip", batch_sample_sizes=None, device_id: int = 0,

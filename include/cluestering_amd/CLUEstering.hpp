@@ -3,22 +3,27 @@
 // data_structures/PointsHost.hpp:29-257, data_structures/PointsDevice.hpp:32),
 // running on libclue_amd.so through the C ABI in clue_amd.h.
 //
-// Link with -lclue_amd. Single precision (the library's device path is fp32).
+// Link with -lclue_amd. Points are float or double (PointsHost/PointsDevice<Ndim, T>,
+// Clusterer<Ndim, T>); every device stage computes in T.
 #pragma once
 
 #include <algorithm>
 #include <array>
+#include <cmath>
+#include <concepts>
 #include <cstddef>
 #include <cstdint>
 #include <limits>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <ranges>
 #include <span>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -99,26 +104,47 @@ template <class K>
 concept convolutional_kernel = K::is_kernel && requires(const K& k, clue_params& p) { k.fill(p); };
 }  // namespace concepts
 
-// The reference's Queue (utils/get_queue.hpp:19-48): a device and a stream.
+// The reference's Queue (utils/get_queue.hpp:19-48): a device and a stream. The
+// library context behind it is created on first use (copies share it), so host-only
+// work such as PointsHost never touches the device.
 class Queue {
  public:
-  explicit Queue(int device = 0, void* stream = nullptr) {
-    clue_ctx* c = nullptr;
-    detail::check(clue_context_create(device, stream, &c));
-    m_ctx = std::shared_ptr<clue_ctx>(c, [](clue_ctx* p) { clue_context_destroy(p); });
-    m_device = device;
+  explicit Queue(int device = 0, void* stream = nullptr)
+      : m_state{std::make_shared<State>()}, m_device{device} {
+    m_state->stream = stream;
   }
-  clue_ctx* ctx() const { return m_ctx.get(); }
+  clue_ctx* ctx() const {
+    std::call_once(m_state->once, [&] {
+      clue_ctx* c = nullptr;
+      m_state->status = clue_context_create(m_device, m_state->stream, &c);
+      if (m_state->status == CLUE_OK) m_state->ctx = c;
+      else m_state->error = clue_last_error();
+    });
+    if (!m_state->ctx)
+      throw std::runtime_error("could not create a context on device " +
+                               std::to_string(m_device) + ": " + m_state->error);
+    return m_state->ctx;
+  }
   int device() const { return m_device; }
-  void wait() const { detail::check(clue_synchronize(m_ctx.get())); }
+  void wait() const { detail::check(clue_synchronize(ctx())); }
   void* stream() const {
     void* s = nullptr;
-    detail::check(clue_context_stream(m_ctx.get(), &s));
+    detail::check(clue_context_stream(ctx(), &s));
     return s;
   }
 
  private:
-  std::shared_ptr<clue_ctx> m_ctx;
+  struct State {
+    std::once_flag once;
+    clue_ctx* ctx = nullptr;
+    void* stream = nullptr;
+    clue_status status = CLUE_OK;
+    std::string error;
+    ~State() {
+      if (ctx) clue_context_destroy(ctx);
+    }
+  };
+  std::shared_ptr<State> m_state;
   int m_device = 0;
 };
 
@@ -185,14 +211,45 @@ class GaussianKernel {
 };
 
 // ---- Distance metrics (core/DistanceMetrics.hpp:64-371) ----------------------------------
+// Each metric carries its parameters to the device (fill) and can be evaluated on
+// the host on two points, coordinates first and the weight last (Point<Ndim, T>,
+// DistanceMetrics.hpp:48-49). Sums run as the reference's right fold
+// d0 + (d1 + (... + 0)) (internal/meta/accumulate.hpp), maxima as the running
+// std::max from 0 (internal/meta/maximum.hpp); the device stages evaluate the
+// same expressions in the same order.
+template <std::size_t Ndim, class T = float>
+using Point = std::array<T, Ndim + 1>;
+
+namespace detail {
+template <std::size_t Ndim, class T, class F>
+T right_fold_sum(F&& term) {
+  T acc = T(0);
+  for (std::size_t k = Ndim; k-- > 0;) acc = term(k) + acc;
+  return acc;
+}
+template <std::size_t Ndim, class T, class F>
+T running_max(F&& term) {
+  T m = T(0);
+  for (std::size_t k = 0; k < Ndim; ++k) m = std::max(m, term(k));
+  return m;
+}
+}  // namespace detail
+
 template <std::size_t Ndim, class T = float>
 struct EuclideanMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   template <class P>
   void fill(P& p) const { p.metric = CLUE_METRIC_EUCLIDEAN; }
+  // core/DistanceMetrics.hpp:108-113
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return std::sqrt(detail::right_fold_sum<Ndim, T>(
+        [&](std::size_t k) { return (a[k] - b[k]) * (a[k] - b[k]); }));
+  }
 };
 template <std::size_t Ndim, class T = float>
 struct WeightedEuclideanMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   std::array<T, Ndim> weights;
   WeightedEuclideanMetric(const std::array<T, Ndim>& w) : weights{w} {}
@@ -204,9 +261,15 @@ struct WeightedEuclideanMetric {
     p.metric = CLUE_METRIC_WEIGHTED_EUCLIDEAN;
     for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(weights[d]);
   }
+  // core/DistanceMetrics.hpp:156-162: (w * diff) * diff per dimension.
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return std::sqrt(detail::right_fold_sum<Ndim, T>(
+        [&](std::size_t k) { return weights[k] * (a[k] - b[k]) * (a[k] - b[k]); }));
+  }
 };
 template <std::size_t Ndim, class T = float>
 struct PeriodicEuclideanMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   std::array<T, Ndim> periods;
   PeriodicEuclideanMetric(const std::array<T, Ndim>& per) : periods{per} {}
@@ -218,21 +281,40 @@ struct PeriodicEuclideanMetric {
     p.metric = CLUE_METRIC_PERIODIC_EUCLIDEAN;
     for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(periods[d]);
   }
+  // core/DistanceMetrics.hpp:212-220: min(|diff|, period - |diff|) per dimension.
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return std::sqrt(detail::right_fold_sum<Ndim, T>([&](std::size_t k) {
+      const T diff = std::fabs(a[k] - b[k]);
+      const T pd = std::min(diff, periods[k] - diff);
+      return pd * pd;
+    }));
+  }
 };
 template <std::size_t Ndim, class T = float>
 struct ManhattanMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   template <class P>
   void fill(P& p) const { p.metric = CLUE_METRIC_MANHATTAN; }
+  // core/DistanceMetrics.hpp:241-245
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return detail::right_fold_sum<Ndim, T>([&](std::size_t k) { return std::fabs(a[k] - b[k]); });
+  }
 };
 template <std::size_t Ndim, class T = float>
 struct ChebyshevMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   template <class P>
   void fill(P& p) const { p.metric = CLUE_METRIC_CHEBYSHEV; }
+  // core/DistanceMetrics.hpp:266-270
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return detail::running_max<Ndim, T>([&](std::size_t k) { return std::fabs(a[k] - b[k]); });
+  }
 };
 template <std::size_t Ndim, class T = float>
 struct WeightedChebyshevMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   std::array<T, Ndim> weights;
   WeightedChebyshevMetric(const std::array<T, Ndim>& w) : weights{w} {}
@@ -244,12 +326,23 @@ struct WeightedChebyshevMetric {
     p.metric = CLUE_METRIC_WEIGHTED_CHEBYSHEV;
     for (std::size_t d = 0; d < Ndim; ++d) p.mparam[d] = static_cast<detail::param_t<P>>(weights[d]);
   }
+  // core/DistanceMetrics.hpp:313-318
+  T operator()(const Point<Ndim, T>& a, const Point<Ndim, T>& b) const {
+    return detail::running_max<Ndim, T>(
+        [&](std::size_t k) { return weights[k] * std::fabs(a[k] - b[k]); });
+  }
 };
+template <std::size_t Ndim, class T>
+class PointsHost;
 template <std::size_t Ndim, class T = float>
 struct MahalanobisMetric {
+  using value_type = T;
   static constexpr bool is_metric = true;
   template <class P>
   void fill(P& p) const { p.metric = CLUE_METRIC_MAHALANOBIS; }
+  // core/DistanceMetrics.hpp:75-87: diff^2 / (sigma_i^2 + sigma_j^2) per dimension,
+  // over host points whose sigma is set in every dimension.
+  T operator()(const PointsHost<Ndim, T>& points, std::size_t i, std::size_t j) const;
 };
 
 namespace metrics {
@@ -360,8 +453,25 @@ class AssociationMapDevice {
 };
 
 // ---- Points ------------------------------------------------------------------------------
+namespace detail {
+template <class P>
+concept pointer = std::is_pointer_v<P>;
+
+inline void check_buffer(std::size_t have, std::size_t need) {
+  if (have < need) throw std::invalid_argument("buffer too small for the number of points");
+}
+inline int32_t check_points(int32_t n, const char* who) {
+  // detail/HostViewPartition.hpp:16-19, detail/DeviceViewPartition.hpp:13-19
+  if (n <= 0)
+    throw std::invalid_argument(std::string("Number of points passed to ") + who +
+                                " constructor must be positive.");
+  return n;
+}
+}  // namespace detail
+
 // Host SoA (detail/HostViewPartition.hpp:15-21): Ndim coordinate columns, weights, and
-// the cluster indexes, in one owned buffer or in caller buffers.
+// the cluster indexes, in one owned buffer or in caller buffers. The eight
+// constructor forms are those of data_structures/PointsHost.hpp:59-137.
 template <std::size_t Ndim, class T = float>
 class PointsHost {
   static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
@@ -369,28 +479,62 @@ class PointsHost {
 
  public:
   using value_type = T;
+  using element_type = T;
 
-  PointsHost(Queue&, int32_t n) : m_n{check_n(n)}, m_own((Ndim + 1) * static_cast<std::size_t>(n)),
-                                  m_own_idx(static_cast<std::size_t>(n), 0) {
-    for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = m_own.data() + d * n;
-    m_weight = m_own.data() + Ndim * static_cast<std::size_t>(n);
-    m_cluster = m_own_idx.data();
+  // Bytes of the one-buffer layout [x_0 .. x_{Ndim-1}][weight][cluster_index]
+  // (detail/HostViewPartition.hpp:13-20).
+  static std::size_t soa_bytes(int32_t n) {
+    return ((Ndim + 1) * sizeof(T) + sizeof(int)) * static_cast<std::size_t>(n);
+  }
+
+  // Owned storage.
+  PointsHost(Queue&, int32_t n)
+      : m_n{detail::check_points(n, "PointsHost")}, m_own(soa_bytes(n)) {
+    partition(m_own.data());
+  }
+  // One caller buffer holding the whole SoA, soa_bytes(n) long.
+  PointsHost(Queue&, int32_t n, std::span<std::byte> buffer)
+      : m_n{detail::check_points(n, "PointsHost")} {
+    detail::check_buffer(buffer.size(), soa_bytes(n));
+    partition(buffer.data());
   }
   // (Ndim+1) x n SoA input plus an n-int output (the Python binding's layout, Run.hpp:39).
-  PointsHost(Queue&, int32_t n, T* input, int* output) : m_n{check_n(n)} {
+  PointsHost(Queue& q, int32_t n, std::span<T> input, std::span<int> output)
+      : PointsHost(q, n, input.data(), output.data()) {
+    detail::check_buffer(input.size(), (Ndim + 1) * std::size_t(n));
+    detail::check_buffer(output.size(), std::size_t(n));
+  }
+  // Ndim x n coordinates, n weights, n-int output.
+  PointsHost(Queue& q, int32_t n, std::span<T> coordinates, std::span<T> weights,
+             std::span<int> output)
+      : PointsHost(q, n, coordinates.data(), weights.data(), output.data()) {
+    detail::check_buffer(coordinates.size(), Ndim * std::size_t(n));
+    detail::check_buffer(weights.size(), std::size_t(n));
+    detail::check_buffer(output.size(), std::size_t(n));
+  }
+  // One range per coordinate, then the weights and the output.
+  template <std::ranges::contiguous_range... R>
+    requires(sizeof...(R) == Ndim + 2 && Ndim > 1)
+  PointsHost(Queue&, int32_t n, R&&... buffers) : m_n{detail::check_points(n, "PointsHost")} {
+    (detail::check_buffer(std::ranges::size(buffers), std::size_t(n)), ...);
+    columns(std::ranges::data(buffers)...);
+  }
+  PointsHost(Queue&, int32_t n, T* input, int* output) : m_n{detail::check_points(n, "PointsHost")} {
     for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = input + d * n;
     m_weight = input + Ndim * static_cast<std::size_t>(n);
     m_cluster = output;
   }
-  PointsHost(Queue& q, int32_t n, std::span<T> input, std::span<int> output)
-      : PointsHost(q, n, input.data(), output.data()) {
-    if (input.size() < (Ndim + 1) * static_cast<std::size_t>(n) || output.size() < static_cast<std::size_t>(n))
-      throw std::invalid_argument("buffers too small for the number of points");
-  }
-  PointsHost(Queue&, int32_t n, T* coordinates, T* weights, int* output) : m_n{check_n(n)} {
+  PointsHost(Queue&, int32_t n, T* coordinates, T* weights, int* output)
+      : m_n{detail::check_points(n, "PointsHost")} {
     for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = coordinates + d * n;
     m_weight = weights;
     m_cluster = output;
+  }
+  // One pointer per coordinate, then the weights and the output.
+  template <detail::pointer... P>
+    requires(sizeof...(P) == Ndim + 2 && Ndim > 1)
+  PointsHost(Queue&, int32_t n, P... buffers) : m_n{detail::check_points(n, "PointsHost")} {
+    columns(buffers...);
   }
   PointsHost(const PointsHost&) = delete;
   PointsHost& operator=(const PointsHost&) = delete;
@@ -412,6 +556,13 @@ class PointsHost {
   std::span<int> clusterIndexes() { return {m_cluster, static_cast<std::size_t>(m_n)}; }
   bool clustered() const { return m_clustered; }
   void mark_clustered() { m_clustered = true; }
+  // Point i as coordinates then weight (the argument of the metrics' call operator).
+  Point<Ndim, T> operator[](std::size_t i) const {
+    Point<Ndim, T> p{};
+    for (std::size_t d = 0; d < Ndim; ++d) p[d] = m_coords[d][i];
+    p[Ndim] = m_weight[i];
+    return p;
+  }
 
   std::size_t n_clusters() const {
     int32_t m = -1;
@@ -437,17 +588,28 @@ class PointsHost {
   const std::uint32_t* tags_ptr() const { return m_tags; }
 
  private:
-  static int32_t check_n(int32_t n) {
-    if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
-    return n;
+  void partition(std::byte* b) {
+    const std::size_t nn = static_cast<std::size_t>(m_n);
+    T* f = reinterpret_cast<T*>(b);
+    for (std::size_t d = 0; d < Ndim; ++d) m_coords[d] = f + d * nn;
+    m_weight = f + Ndim * nn;
+    m_cluster = reinterpret_cast<int*>(f + (Ndim + 1) * nn);
+  }
+  template <class... P>
+  void columns(P... p) {
+    auto t = std::make_tuple(p...);
+    [&]<std::size_t... D>(std::index_sequence<D...>) {
+      ((m_coords[D] = std::get<D>(t)), ...);
+    }(std::make_index_sequence<Ndim>{});
+    m_weight = std::get<Ndim>(t);
+    m_cluster = std::get<Ndim + 1>(t);
   }
   std::span<T> check_span(std::span<T> s) const {
     if (s.size() < static_cast<std::size_t>(m_n)) throw std::invalid_argument("array too short");
     return s;
   }
   int32_t m_n;
-  std::vector<T> m_own;
-  std::vector<int> m_own_idx;
+  std::vector<std::byte> m_own;
   std::array<T*, Ndim> m_coords{};
   T* m_weight = nullptr;
   int* m_cluster = nullptr;
@@ -457,15 +619,34 @@ class PointsHost {
   bool m_clustered = false;
 };
 
+template <std::size_t Ndim, class T>
+T MahalanobisMetric<Ndim, T>::operator()(const PointsHost<Ndim, T>& points, std::size_t i,
+                                         std::size_t j) const {
+  return std::sqrt(detail::right_fold_sum<Ndim, T>([&](std::size_t k) {
+    const T* s = points.sigma_ptr(k);
+    if (!s) throw std::invalid_argument("sigma must be set in every dimension");
+    const T diff = points.coord_ptr(k)[i] - points.coord_ptr(k)[j];
+    return diff * diff / (s[i] * s[i] + s[j] * s[j]);
+  }));
+}
+
 // Device SoA (detail/DeviceViewPartition.hpp:14-20): coordinates, weights, and the
 // rho / nearest_higher / is_seed / cluster_index results; optional sigma, density
-// uncertainty and tags. All device memory, owned.
+// uncertainty and tags. The seven constructor forms are those of
+// data_structures/PointsDevice.hpp:51-122: owned storage, one caller device
+// buffer with the whole layout, or caller device buffers for the inputs and the
+// cluster indexes, in which case is_seed, rho and nearest_higher live in storage
+// these points own (as the reference's partitionSoAView with an alloc buffer).
+// delta, which the reference does not keep, is always owned here.
 template <std::size_t Ndim, class T = float>
 class PointsDevice {
   static_assert(std::is_same_v<T, float> || std::is_same_v<T, double>,
                 "points are float or double");
 
  public:
+  using value_type = T;
+  using element_type = T;
+
   struct View {
     std::array<T*, Ndim> m_coords;
     T* m_weight;
@@ -483,20 +664,67 @@ class PointsDevice {
     int32_t size() const { return m_n; }
   };
 
-  PointsDevice(Queue& q, int32_t n) : m_q{q}, m_n{n} {
-    if (n <= 0) throw std::invalid_argument("The number of points must be positive.");
-    const std::size_t nn = static_cast<std::size_t>(n);
-    m_block = alloc<T>((Ndim + 3) * nn);
-    m_ints = alloc<int32_t>(3 * nn);
-    for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = m_block.get() + d * nn;
-    m_view.m_weight = m_block.get() + Ndim * nn;
-    m_view.m_rho = m_block.get() + (Ndim + 1) * nn;
-    m_view.m_delta = m_block.get() + (Ndim + 2) * nn;
-    m_view.m_cluster_index = m_ints.get();
-    m_view.m_is_seed = m_ints.get() + nn;
-    m_view.m_nearest_higher = m_ints.get() + 2 * nn;
-    m_view.m_n = n;
+  // Bytes of the one-buffer layout [x_0 .. x_{Ndim-1}][weight][cluster_index][is_seed]
+  // [rho][nearest_higher] (detail/DeviceViewPartition.hpp:13-40).
+  static std::size_t soa_bytes(int32_t n) {
+    return ((Ndim + 2) * sizeof(T) + 3 * sizeof(int32_t)) * static_cast<std::size_t>(n);
   }
+
+  PointsDevice(Queue& q, int32_t n) : m_q{q}, m_n{detail::check_points(n, "PointsDevice")} {
+    m_block = alloc<std::byte>(soa_bytes(n));
+    partition(m_block.get());
+    own_delta();
+  }
+  PointsDevice(Queue& q, int32_t n, std::span<std::byte> buffer)
+      : m_q{q}, m_n{detail::check_points(n, "PointsDevice")} {
+    detail::check_buffer(buffer.size(), soa_bytes(n));
+    partition(buffer.data());
+    own_delta();
+  }
+  PointsDevice(Queue& q, int32_t n, std::span<T> input, std::span<int> output)
+      : PointsDevice(q, n, input.data(), output.data()) {
+    detail::check_buffer(input.size(), (Ndim + 1) * std::size_t(n));
+    detail::check_buffer(output.size(), std::size_t(n));
+  }
+  PointsDevice(Queue& q, int32_t n, std::span<T> coordinates, std::span<T> weights,
+               std::span<int> output)
+      : PointsDevice(q, n, coordinates.data(), weights.data(), output.data()) {
+    detail::check_buffer(coordinates.size(), Ndim * std::size_t(n));
+    detail::check_buffer(weights.size(), std::size_t(n));
+    detail::check_buffer(output.size(), std::size_t(n));
+  }
+  PointsDevice(Queue& q, int32_t n, T* input, int* output)
+      : m_q{q}, m_n{detail::check_points(n, "PointsDevice")} {
+    for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = input + d * std::size_t(n);
+    m_view.m_weight = input + Ndim * std::size_t(n);
+    m_view.m_cluster_index = output;
+    own_results();
+  }
+  PointsDevice(Queue& q, int32_t n, T* coordinates, T* weights, int* output)
+      : m_q{q}, m_n{detail::check_points(n, "PointsDevice")} {
+    for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = coordinates + d * std::size_t(n);
+    m_view.m_weight = weights;
+    m_view.m_cluster_index = output;
+    own_results();
+  }
+  // One device pointer per coordinate, then the weights and the cluster indexes.
+  template <detail::pointer... P>
+    requires(sizeof...(P) == Ndim + 2 && Ndim > 1)
+  PointsDevice(Queue& q, int32_t n, P... buffers)
+      : m_q{q}, m_n{detail::check_points(n, "PointsDevice")} {
+    auto t = std::make_tuple(buffers...);
+    [&]<std::size_t... D>(std::index_sequence<D...>) {
+      ((m_view.m_coords[D] = std::get<D>(t)), ...);
+    }(std::make_index_sequence<Ndim>{});
+    m_view.m_weight = std::get<Ndim>(t);
+    m_view.m_cluster_index = std::get<Ndim + 1>(t);
+    own_results();
+  }
+  PointsDevice(const PointsDevice&) = delete;
+  PointsDevice& operator=(const PointsDevice&) = delete;
+  PointsDevice(PointsDevice&&) = default;
+  PointsDevice& operator=(PointsDevice&&) = default;
+
   int32_t size() const { return m_n; }
   const View& view() const { return m_view; }
   std::span<T> coords(std::size_t d) const {
@@ -577,10 +805,39 @@ class PointsDevice {
     return std::unique_ptr<U, detail::DeviceDeleter<U>>(static_cast<U*>(p),
                                                         detail::DeviceDeleter<U>{m_q.ctx()});
   }
+  // [x_0 .. x_{Ndim-1}][weight][cluster_index][is_seed][rho][nearest_higher] from b.
+  void partition(std::byte* b) {
+    const std::size_t nn = static_cast<std::size_t>(m_n);
+    T* f = reinterpret_cast<T*>(b);
+    for (std::size_t d = 0; d < Ndim; ++d) m_view.m_coords[d] = f + d * nn;
+    m_view.m_weight = f + Ndim * nn;
+    std::byte* r = b + (Ndim + 1) * sizeof(T) * nn;
+    m_view.m_cluster_index = reinterpret_cast<int32_t*>(r);
+    m_view.m_is_seed = reinterpret_cast<int32_t*>(r + sizeof(int32_t) * nn);
+    m_view.m_rho = reinterpret_cast<T*>(r + 2 * sizeof(int32_t) * nn);
+    m_view.m_nearest_higher = reinterpret_cast<int32_t*>(r + 2 * sizeof(int32_t) * nn + sizeof(T) * nn);
+    m_view.m_n = m_n;
+  }
+  void own_delta() {
+    m_delta_own = alloc<T>(std::size_t(m_n));
+    m_view.m_delta = m_delta_own.get();
+  }
+  // is_seed, rho, nearest_higher (and delta) when the caller gives only inputs and
+  // cluster indexes.
+  void own_results() {
+    const std::size_t nn = static_cast<std::size_t>(m_n);
+    m_block = alloc<std::byte>((2 * sizeof(T) + 2 * sizeof(int32_t)) * nn);
+    T* f = reinterpret_cast<T*>(m_block.get());
+    m_view.m_rho = f;
+    m_view.m_delta = f + nn;
+    m_view.m_is_seed = reinterpret_cast<int32_t*>(f + 2 * nn);
+    m_view.m_nearest_higher = m_view.m_is_seed + nn;
+    m_view.m_n = m_n;
+  }
   Queue m_q;
   int32_t m_n;
-  std::unique_ptr<T, detail::DeviceDeleter<T>> m_block;
-  std::unique_ptr<int32_t, detail::DeviceDeleter<int32_t>> m_ints;
+  std::unique_ptr<std::byte, detail::DeviceDeleter<std::byte>> m_block{nullptr, {nullptr}};
+  std::unique_ptr<T, detail::DeviceDeleter<T>> m_delta_own{nullptr, {nullptr}};
   std::unique_ptr<T, detail::DeviceDeleter<T>> m_sig_own, m_u_own;
   std::unique_ptr<std::uint32_t, detail::DeviceDeleter<std::uint32_t>> m_tags_own;
   View m_view{};

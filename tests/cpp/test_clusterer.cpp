@@ -4,6 +4,7 @@
 // wrapper tests/test_cpp_api.py runs the built binary.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <numbers>
 #include <string>
 #include <vector>
@@ -201,6 +202,77 @@ static void test_device_points_and_clusters() {
   std::remove(out.c_str());
 }
 
+static void test_device_buffer_forms() {
+  // PointsDevice over caller device memory (data_structures/PointsDevice.hpp:51-122):
+  // every form gives the labels and densities of the owning form, and the one-buffer
+  // form writes its results where the reference's layout puts them.
+  auto q = clue::get_queue(0u);
+  auto h = clue::read_csv<2, float>(q, data_dir + "/data_4096.csv");
+  const int32_t n = h.size();
+  const std::size_t nn = std::size_t(n);
+  clue::Clusterer<2> algo(q, 1.5f, 10.f, 1.5f);
+  clue::PointsDevice<2> own(q, n);
+  clue::copyToDevice(q, own, h);
+  algo.make_clusters(q, own);
+  std::vector<int32_t> want(nn), got(nn);
+  std::vector<float> want_rho(nn), got_rho(nn);
+  clue::detail::check(clue_memcpy_d2h(q.ctx(), want.data(), own.view().m_cluster_index, 4 * nn));
+  clue::detail::check(clue_memcpy_d2h(q.ctx(), want_rho.data(), own.view().m_rho, 4 * nn));
+  CHECK(own.n_clusters() == 20);
+
+  void* mem = nullptr;
+  const std::size_t bytes = clue::PointsDevice<2>::soa_bytes(n);
+  clue::detail::check(clue_malloc(q.ctx(), bytes, &mem));
+  std::byte* buf = static_cast<std::byte*>(mem);
+  float* fbuf = static_cast<float*>(mem);
+  auto run_and_compare = [&](clue::PointsDevice<2>& d) {
+    clue::detail::check(clue_memset_d8(q.ctx(), d.view().m_cluster_index, 0xff, 4 * nn));
+    clue::copyToDevice(q, d, h);
+    algo.make_clusters(q, d);
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), got.data(), d.view().m_cluster_index, 4 * nn));
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), got_rho.data(), d.view().m_rho, 4 * nn));
+    return got == want && std::memcmp(got_rho.data(), want_rho.data(), 4 * nn) == 0;
+  };
+  {
+    clue::PointsDevice<2> d(q, n, std::span<std::byte>(buf, bytes));
+    CHECK(run_and_compare(d));
+    // [x][y][w][cluster_index][is_seed][rho][nearest_higher]
+    std::vector<int32_t> at(nn);
+    std::vector<float> rho_at(nn);
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), at.data(), buf + 3 * 4 * nn, 4 * nn));
+    clue::detail::check(clue_memcpy_d2h(q.ctx(), rho_at.data(), buf + 5 * 4 * nn, 4 * nn));
+    CHECK(at == want);
+    CHECK(std::memcmp(rho_at.data(), want_rho.data(), 4 * nn) == 0);
+    CHECK_THROWS(clue::PointsDevice<2>(q, n, std::span<std::byte>(buf, bytes - 1)));
+  }
+  int32_t* out = reinterpret_cast<int32_t*>(fbuf + 3 * nn);
+  {
+    clue::PointsDevice<2> d(q, n, std::span<float>(fbuf, 3 * nn), std::span<int>(out, nn));
+    CHECK(run_and_compare(d));
+    CHECK(d.view().m_cluster_index == out && d.view().m_weight == fbuf + 2 * nn);
+  }
+  {
+    clue::PointsDevice<2> d(q, n, fbuf, out);
+    CHECK(run_and_compare(d));
+  }
+  {
+    clue::PointsDevice<2> d(q, n, std::span<float>(fbuf, 2 * nn), std::span<float>(fbuf + 2 * nn, nn),
+                            std::span<int>(out, nn));
+    CHECK(run_and_compare(d));
+  }
+  {
+    clue::PointsDevice<2> d(q, n, fbuf, fbuf + 2 * nn, out);
+    CHECK(run_and_compare(d));
+  }
+  {
+    clue::PointsDevice<2> d(q, n, fbuf, fbuf + nn, fbuf + 2 * nn, out);
+    CHECK(run_and_compare(d));
+    CHECK(d.view().m_coords[1] == fbuf + nn);
+  }
+  CHECK_THROWS(clue::PointsDevice<2>(q, 0));
+  clue::detail::check(clue_free(q.ctx(), mem));
+}
+
 static void test_precisions_and_silhouette() {
   // tests/test_clustering.cpp:14-57, 94-130: single and double precision on the
   // benchmark and sissa fixtures, judged by the silhouette score.
@@ -274,6 +346,7 @@ int main(int argc, char** argv) {
     test_periodic();
     test_batched();
     test_device_points_and_clusters();
+    test_device_buffer_forms();
     test_precisions_and_silhouette();
   } catch (const std::exception& e) {
     std::printf("FAIL exception: %s\n", e.what());

@@ -60,6 +60,45 @@ def test_choose_kernel_and_metric():
         c.choose_metric("cosine")
 
 
+def test_read_data_types_and_layout():
+    c = clue.clusterer(1.5, 10.0)
+    c.read_data([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0], [1.0, 1.0, 2.0]])
+    assert c.clust_data.coords.dtype == np.float64  # Python floats keep double
+    assert c.clust_data.coords.flags.c_contiguous and c.clust_data.coords.shape == (3, 3)
+    np.testing.assert_array_equal(c.weight, [1.0, 1.0, 2.0])
+    assert c.clust_data.results.dtype == np.int32 and c.n_dim == 2 and c.n_points == 3
+    x = np.arange(4, dtype=np.float32)
+    c.read_data(np.stack([x, 2 * x, np.ones(4, np.float32)]))
+    assert c.clust_data.coords.dtype == np.float32
+    c.read_data({"a": [1, 2], "b": [3, 4], "weight": [1, 1]})
+    assert c.clust_data.coords.dtype == np.float32  # dicts and frames read as float32
+    np.testing.assert_array_equal(c.clust_data.coords, [[1, 2], [3, 4], [1, 1]])
+    with pytest.raises(ValueError):
+        c.read_data([[1.0, 2.0], [1.0]])  # ragged rows
+    with pytest.raises(ValueError):
+        c.read_data(pd.DataFrame({"weight": [1.0]}))
+    with pytest.raises(ValueError):
+        c.read_data(3.0)
+    c.read_data([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0], [7.0, 8.0]])
+    part = c._partial_dimension_dataset([2, 0])
+    np.testing.assert_array_equal(part, [[5.0, 6.0], [1.0, 2.0], [7.0, 8.0]])
+    assert part.flags.c_contiguous and not np.shares_memory(part, c.clust_data.coords)
+
+
+def test_kernel_and_metric_messages():
+    c = clue.clusterer(1.5, 10.0)
+    with pytest.raises(ValueError, match="gaussian kernel requires 2 parameters"):
+        c.choose_kernel("gaus", [1.0])
+    with pytest.raises(ValueError, match="flat kernel requires 1 parameter"):
+        c.choose_kernel("flat", [])
+    with pytest.raises(ValueError, match="per-dimension periods"):
+        c.choose_metric("periodic_euclidean")
+    c.choose_metric("weighted_chebyshev", [1.0, 2.0])
+    assert c._metric.resolve(2)[1] == (1.0, 2.0) or list(c._metric.resolve(2)[1])[:2] == [1.0, 2.0]
+    c.choose_kernel("flat", [0.25])
+    assert c._kernel.flat == 0.25
+
+
 def test_defaults_follow_reference():
     c = clue.clusterer(1.3, 5.0)
     assert c._outlier_distance == 1.3 and c._seeding_distance == 1.3
