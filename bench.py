@@ -11,8 +11,13 @@ resident in HBM). Printed as one JSON line, with:
     events, and the batched/per-event speed-up;
   - per-stage device times (hipEvents between the stages) and the roofline of
     the dominant stage (algorithmic bytes / average duration vs 8 TB/s);
-  - the CPU baseline: the serial oracle (a restatement of the reference's serial
-    backend) timed on a bounded sample of the same events on this host;
+  - the CPU baseline: the oracle (a restatement of the reference's serial
+    backend), OpenMP over all host threads and serial, timed on a bounded sample
+    of the same events on this host;
+  - the latency of one batch (p50/p99 over launches waited for one at a time);
+  - the neighbour stages' work units (pairs within the radius, counted once by the
+    oracle) per second, and the VALU issue rate of the dominant kernel;
+  - the served stream swept over offered rates (p50/p99 from arrival to labels);
   - bit-exactness of the GPU labels against that oracle sample, and of the
     per-event path against the batched one.
 """
@@ -31,6 +36,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+N_CUS = 256
+CLOCK_HZ = 2.4e9
+VALU_PEAK_PER_CU_CYCLE = 2.0
 METRIC = "events/s + p50/p99 latency, batched vs per-event GPU CLUE, bit-exact labels"
 
 
@@ -107,14 +115,16 @@ def _oracle_cfg(oracle, w):
 
 
 def cpu_baseline(w, offsets, gpu_labels, seconds):
-    """Serial oracle over the events in order until `seconds` of CPU work; returns
-    events/s, the sample, and whether the GPU labels of the sample are bit-exact
-    (batched ids = per-event ids + seeds of the previous events)."""
+    """Serial oracle over the events in order until `seconds` of CPU work, then the
+    OpenMP oracle over the same events; returns both rates, the sample, and whether
+    the GPU labels of the sample are bit-exact (batched ids = per-event ids + seeds
+    of the previous events; the OpenMP labels must equal the serial ones)."""
     from oracle import oracle
     cfg = _oracle_cfg(oracle, w)
-    done, exact, t_cpu = 0, True, 0.0
+    done, exact, t_cpu, t_omp = 0, True, 0.0, 0.0
     first_id = 0
     points = 0
+    serial_labels = []
     for e, n in enumerate(w.sizes):
         o, n = int(offsets[e]), int(n)
         c = np.ascontiguousarray(w.coords[:, o:o + n])
@@ -123,12 +133,23 @@ def cpu_baseline(w, offsets, gpu_labels, seconds):
         t_cpu += time.perf_counter() - t0
         want = np.where(r.cluster >= 0, r.cluster + first_id, -1)
         exact = exact and bool(np.array_equal(gpu_labels[o:o + n], want))
+        serial_labels.append(r.cluster)
         first_id += r.n_seeds
         done += 1
         points += n
         if t_cpu >= seconds:
             break
-    return done / t_cpu, done, points, t_cpu, exact
+    omp_same = True
+    for e in range(done):
+        o, n = int(offsets[e]), int(w.sizes[e])
+        c = np.ascontiguousarray(w.coords[:, o:o + n])
+        t0 = time.perf_counter()
+        r = oracle.run(cfg, c, w.weight[o:o + n], omp=True)
+        t_omp += time.perf_counter() - t0
+        omp_same = omp_same and bool(np.array_equal(r.cluster, serial_labels[e]))
+    return {"serial_eps": done / t_cpu, "omp_eps": done / t_omp, "events": done, "points": points,
+            "serial_s": t_cpu, "omp_s": t_omp, "exact": exact and omp_same,
+            "omp_threads": oracle.num_threads(True)}
 
 
 def main():
@@ -141,8 +162,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-event", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-sweep", action="store_true",
-                    help="also sweep the served stream over 1k..50k events/s")
+    ap.add_argument("--serve-rates", default="1000,2000,5000,10000,20000,50000",
+                    help="offered rates (events/s) of the served stream, comma separated")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
@@ -191,6 +212,16 @@ def main():
           file=sys.stderr, flush=True)
     batched_labels = pts.cluster_index.to_host()
 
+    # Latency of one batch: each launch waited for on its own (launch to completion
+    # on the host clock), over as many launches as the timed loop.
+    lat_ms = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        plan.launch()
+        ctx.synchronize()
+        lat_ms.append((time.perf_counter() - ts) * 1e3)
+    lat_ms = np.asarray(lat_ms)
+
     # Per-stage device times: hipEvents between the stages on the context stream, on
     # the same kernels run eagerly (same batch, same launches as the graph).
     acc = {}
@@ -213,15 +244,40 @@ def main():
     dom = max(alg, key=lambda k: acc.get(k, 0.0))
     dom_us = acc[dom]
     achieved = alg[dom] / (dom_us * 1e-6) / 1e9
-    traffic = None
+    # HBM bytes per launch and VALU instructions per launch of this kernel, from the
+    # committed rocprofv3 counter runs of the same workload.
+    prof = {}
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{w.name}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5), "traffic": traffic,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 5),
+                "traffic": prof.get("hbm_bytes_per_launch"),
                 "kernel": f"k_{dom}", "alg_bytes_per_launch": alg[dom],
                 "avg_launch_us": round(dom_us, 2)}
+    valu = prof.get("valu_insts_per_launch")
+    if valu:
+        # Issue rate against 2 wave64 FP32 VALU instructions per CU per cycle
+        # (157.3 TFLOP/s = 256 CUs x 2.4 GHz x 2 x 64 lanes x 2 flops).
+        per_cu_cycle = valu / (dom_us * 1e-6) / (N_CUS * CLOCK_HZ)
+        roofline["valu"] = {"insts_per_launch": valu, "issue_per_cu_cycle": round(per_cu_cycle, 3),
+                            "peak_per_cu_cycle": VALU_PEAK_PER_CU_CYCLE,
+                            "frac": round(per_cu_cycle / VALU_PEAK_PER_CU_CYCLE, 4),
+                            "source": prof.get("source")}
+        roofline["binding"] = "valu" if per_cu_cycle / VALU_PEAK_PER_CU_CYCLE > \
+            achieved / PEAK_HBM_GBS else "hbm"
+    pairs = None
+    pfile = os.path.join(ROOT, "tests", "golden", f"pair_counts_{w.name}.json")
+    if os.path.exists(pfile):
+        with open(pfile) as f:
+            pc = json.load(f)
+        pairs = {"W_b": pc["W_b"], "W_c": pc["W_c"],
+                 "density_pairs_per_s": round(pc["W_b"] / (acc["density"] * 1e-6), 1),
+                 "nh_pairs_per_s": round(pc["W_c"] / (acc["nearest_higher"] * 1e-6), 1),
+                 "density_visited": pc["visited_density"], "nh_visited": pc["visited_nh"]}
+        if valu and dom == "density":
+            pairs["density_lane_insts_per_pair"] = round(valu * 64 / pc["W_b"], 1)
     pipeline_bytes = w.n * (88 if D == 2 else 104)
     pipeline_us = sum(acc.values())
 
@@ -235,8 +291,12 @@ def main():
         "roofline": roofline,
         "stage_us": stage_us,
         "pipeline_hbm_frac": round(pipeline_bytes / (pipeline_us * 1e-6) / 1e9 / PEAK_HBM_GBS, 5),
-        "batched_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4)},
+        "batched_latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4),
+                               "p99": round(float(np.percentile(lat_ms, 99)), 4),
+                               "launches": int(lat_ms.size)},
     }
+    if pairs:
+        out["pairs"] = pairs
 
     if not args.no_per_event:
         total, lat, labels = _per_event_run(ctx, clus, pts, w, offsets)
@@ -258,16 +318,19 @@ def main():
         print(f"[bench] per-event path: {pe:.1f} events/s", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
-        from oracle import oracle
-        eps, done, npts, secs, exact = cpu_baseline(w, offsets, batched_labels, args.cpu_seconds)
+        cb = cpu_baseline(w, offsets, batched_labels, args.cpu_seconds)
+        sample = (f"first {cb['events']} events ({cb['points']} points) of the same batch; "
+                  f"oracle (restatement of the reference's serial backend; the OpenMP build "
+                  f"runs the points of each stage in parallel)")
         out["cpu_baseline"] = {
-            "value": round(eps, 3), "unit": "events/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} events ({npts} points) of the same batch, serial oracle "
-                      f"(restatement of the reference's serial backend), {secs:.1f} s",
-            "omp_threads_available": oracle.num_threads(True)}
-        out["labels_bit_exact"] = {"vs_oracle_sample_events": done, "equal": exact}
-        print(f"[bench] cpu baseline: {eps:.2f} events/s over {done} events", file=sys.stderr,
-              flush=True)
+            "value": round(cb["omp_eps"], 3), "unit": "events/s", "cores": cb["omp_threads"],
+            "kind": "port", "sample": sample + f", OpenMP {cb['omp_s']:.2f} s",
+            "serial": {"value": round(cb["serial_eps"], 3), "cores": 1,
+                       "seconds": round(cb["serial_s"], 2)}}
+        out["labels_bit_exact"] = {"vs_oracle_sample_events": cb["events"], "equal": cb["exact"]}
+        print(f"[bench] cpu baseline: {cb['omp_eps']:.2f} events/s on {cb['omp_threads']} threads, "
+              f"{cb['serial_eps']:.2f} events/s serial, over {cb['events']} events",
+              file=sys.stderr, flush=True)
 
     plan.close()
     ctx.close()
@@ -277,8 +340,7 @@ def main():
         # continuous batching under a 64 GB HBM budget; latency from arrival to labels.
         from cluestering_amd.serving import Server, make_pool, run_stream
         pool = make_pool(256)
-        rates = [1000.0, 2000.0, 5000.0, 10000.0, 20000.0, 50000.0] if args.serve_sweep \
-            else [2000.0]
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
         served = []
         with Server(n_lanes=2, hbm_budget_bytes=64 << 30) as srv:
             run_stream(srv, pool, rate=2000.0, n_events=200, seed=99)  # warm the lane pools

@@ -9,12 +9,10 @@
 namespace clue_dev {
 
 // ---- primitives (clue_primitives.hip) ----
-int64_t scan_temp_elems(int64_t n);
 // Exclusive scan of n int32 (in may equal out). temp: scan_temp_elems(n) ints.
 // total (device, may be NULL) receives the sum.
 void scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* temp, int32_t* total,
                     hipStream_t s);
-int64_t radix_hist_elems(int64_t n);
 void radix_sort_pairs(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n, int key_bits,
                       int32_t* hist, int32_t* scan_temp, bool* in_first, hipStream_t s);
 

@@ -54,6 +54,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total) {
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;
+static_assert(kScanTile == kScanTileElems, "scan workspace sizing");
 
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
                                                         int32_t* __restrict__ partial) {
@@ -111,8 +112,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* in, int32_t
   }
 }
 
-int64_t scan_temp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
-
 void scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* temp, int32_t* total,
                     hipStream_t s) {
   if (n <= 0) return;
@@ -136,6 +135,7 @@ constexpr int kRsBins = 1 << kRsBits;
 constexpr int kRsRounds = 16;
 constexpr int kRsWaveKeys = kWave * kRsRounds;
 constexpr int kRsTile = kRsWaveKeys * kWavesPerBlock;  // 4096
+static_assert(kRsTile == kRsTileKeys && kRsBins == kRsDigitBins, "radix workspace sizing");
 
 __global__ __launch_bounds__(kBlock) void k_rs_hist(const uint32_t* __restrict__ keys, int64_t n,
                                                     int shift, int64_t nblocks,
@@ -202,8 +202,6 @@ __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t* __restric
     }
   }
 }
-
-int64_t radix_hist_elems(int64_t n) { return kRsBins * ((n + kRsTile - 1) / kRsTile); }
 
 // Sorts (keys, vals) by the low `key_bits` bits of the keys. The result ends in
 // (k0, v0) or (k1, v1); *in_first receives true when it is (k0, v0).

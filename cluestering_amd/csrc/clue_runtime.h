@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "clue_amd.h"
+#include "clue_host.h"
 #include "clue_internal.h"
 
 namespace clue_rt {
@@ -25,9 +26,6 @@ const char* last_error();
     if (e_ != hipSuccess)                                                                  \
       return ::clue_rt::fail(CLUE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
-
-constexpr int kChunk = 2048;  // points per chunk of the per-point grids
-constexpr int kDefaultPointsPerTile = 128;
 
 // Device buffer that only grows.
 struct DevBuf {
@@ -47,10 +45,16 @@ struct DevBuf {
       p = nullptr;
       cap = 0;
     }
-    size_t want = bytes + bytes / 4 + 256;
+    const size_t want = grown_capacity(0, bytes);
     hipError_t e = hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
     return e;
+  }
+  // Frees the buffer (capacity back to 0).
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
   }
   template <class T>
   T* as() const {
@@ -58,27 +62,8 @@ struct DevBuf {
   }
 };
 
-// Workspace slots of one batch run.
-enum Slot {
-  S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
-  S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
-  S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_NUM
-};
-
 struct Workspace {
   DevBuf buf[S_NUM];
-};
-
-// Host-side description of one batch (all derived from the event sizes).
-struct BatchPlan {
-  std::vector<EventDesc> events;
-  std::vector<Chunk> chunks;
-  int64_t n_total = 0;
-  int64_t n_tiles_total = 0;
-  int64_t max_items = 0;
-  int key_bits = 0;
-  clue_tuning tuning{};
 };
 
 // The C structs of each precision and the point type they carry.
@@ -95,9 +80,8 @@ struct ValueOf<clue_params_f64> {
 template <class P>
 using value_t = typename ValueOf<P>::type;
 
-// Tuning of a batch: `fixed` when given, else the built-in table's entry for the
-// batch's ndim and mean event size.
-clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events);
+// plan_batch with the error reported through fail(); `fixed` (may be NULL)
+// replaces the built-in table's tuning.
 template <class P>
 clue_status make_batch_plan(const P* p, const uint32_t* sizes, int32_t n_events, BatchPlan* bp,
                             const clue_tuning* fixed = nullptr);
@@ -112,6 +96,9 @@ template <class P, class Pts>
 clue_status check_points(const P* p, const Pts* pts, int64_t n_expected);
 template <class P, class Pts>
 PointArraysT<value_t<P>> point_arrays(const P* p, const Pts* pts);
+// The workspace shape of a batch run (what workspace_requests sizes).
+template <class P, class Pts, class Res>
+WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, const Res* res);
 // Reserves the workspace of one batch, uploads its descriptors on stream s and
 // fills the BatchBuffers and result pointers (NULL results use workspace buffers).
 template <class P, class Pts, class Res>

@@ -9,7 +9,6 @@
 #include <vector>
 
 #include "clue_runtime.h"
-#include "clue_tuning_table.h"
 
 namespace clue_rt {
 
@@ -24,74 +23,12 @@ clue_status fail(clue_status code, const std::string& msg) {
 
 const char* last_error() { return g_last_error.c_str(); }
 
-clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events) {
-  const int64_t mean = n_events > 0 ? n_total / n_events : n_total;
-  for (const TuningEntry& t : kTuningTable)
-    if ((t.ndim == 0 || t.ndim == ndim) && mean <= t.max_mean_event) {
-      clue_tuning out{};
-      out.group_queries = t.group_queries;
-      out.xcd_chunk_log2 = t.xcd_chunk_log2;
-      out.block_waves = t.block_waves;
-      return out;
-    }
-  clue_tuning out{};
-  out.group_queries = 1;
-  out.xcd_chunk_log2 = 4;
-  out.block_waves = 4;
-  return out;
-}
-
 template <class P>
 clue_status make_batch_plan(const P* p, const uint32_t* sizes, int32_t n_events, BatchPlan* bp,
                             const clue_tuning* fixed) {
-  const int D = p->ndim;
-  const int ppt = p->points_per_tile > 0 ? p->points_per_tile : kDefaultPointsPerTile;
-  bp->events.clear();
-  bp->chunks.clear();
-  int64_t off = 0, tiles = 0;
-  for (int32_t e = 0; e < n_events; ++e) {
-    const int64_t n = sizes[e];
-    if (n == 0) continue;
-    // SetupTiles.hpp:30-34: ntiles = ceil(n / ppt); nper = min{k : k^D >= ntiles}.
-    const int64_t nt0 = (n + ppt - 1) / ppt;
-    int64_t nper = 1;
-    auto pw = [D](int64_t b) {
-      int64_t r = 1;
-      for (int k = 0; k < D; ++k) r *= b;
-      return r;
-    };
-    while (pw(nper) < nt0) ++nper;
-    const int64_t nt = pw(nper);
-    EventDesc ev{};
-    ev.p_off = static_cast<int32_t>(off);
-    ev.n = static_cast<int32_t>(n);
-    ev.nper = static_cast<int32_t>(nper);
-    ev.ntiles = static_cast<int32_t>(nt);
-    ev.t_base = static_cast<int32_t>(tiles);
-    ev.index = e;
-    const int32_t eidx = static_cast<int32_t>(bp->events.size());
-    bp->events.push_back(ev);
-    for (int64_t b = 0; b < n; b += kChunk) {
-      Chunk c{};
-      c.event = eidx;
-      c.begin = static_cast<int32_t>(off + b);
-      c.end = static_cast<int32_t>(off + std::min<int64_t>(n, b + kChunk));
-      bp->chunks.push_back(c);
-    }
-    off += n;
-    tiles += nt;
-    if (off > INT32_MAX / 2 || tiles > INT32_MAX / 2)
-      return fail(CLUE_ERR_CAPACITY, "batch too large for 32-bit indexing");
-  }
-  bp->n_total = off;
-  bp->n_tiles_total = tiles;
-  bp->tuning = fixed ? *fixed
-                     : table_tuning(D, off, static_cast<int64_t>(bp->events.size()));
-  bp->max_items = (off + 63) / 64 + tiles;
-  int bits = 0;
-  while ((int64_t(1) << bits) < tiles) ++bits;
-  bp->key_bits = bits;
-  return CLUE_OK;
+  std::string err;
+  const clue_status st = plan_batch(p->ndim, p->points_per_tile, sizes, n_events, fixed, bp, &err);
+  return st == CLUE_OK ? CLUE_OK : fail(st, err);
 }
 
 // Largest t with sqrt(t) <= r (sqrt is correctly rounded on the host and on
@@ -232,6 +169,30 @@ clue_status check_points(const P* p, const Pts* pts, int64_t n_expected) {
 }
 
 
+template <class P, class Pts, class Res>
+WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, const Res* res) {
+  using T = value_t<P>;
+  WorkspaceShape w;
+  w.D = p->ndim;
+  w.value_bytes = sizeof(T);
+  w.key_bytes = sizeof(typename FloatBits<T>::Int);
+  w.n = bp.n_total;
+  w.events = static_cast<int64_t>(bp.events.size());
+  w.chunks = static_cast<int64_t>(bp.chunks.size());
+  w.tiles = bp.n_tiles_total;
+  w.max_items = bp.max_items;
+  w.group_queries = bp.tuning.group_queries != 0;
+  w.nh_blocks = nh_blocks_enabled(p, pts, bp);
+  w.separate_dm_order = p->dm != p->dc && !w.nh_blocks;
+  w.flat_kernel = p->kernel == CLUE_KERNEL_FLAT;
+  w.has_sigma = pts->sigma[0] != nullptr;
+  w.has_u = pts->density_uncertainty != nullptr;
+  w.own_rho = res->rho == nullptr;
+  w.own_nh = res->nearest_higher == nullptr;
+  w.own_is_seed = res->is_seed == nullptr;
+  return w;
+}
+
 // Reserves the workspace of one batch and fills the BatchBuffers. Descriptor
 // uploads are enqueued on the stream (they precede any kernel that reads them).
 template <class P, class Pts, class Res>
@@ -245,62 +206,15 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
   const int64_t E = static_cast<int64_t>(bp.events.size());
   const int64_t C = static_cast<int64_t>(bp.chunks.size());
   const int64_t NT = bp.n_tiles_total;
-  const int64_t hist = radix_hist_elems(n);
-  const int64_t scan_elems =
-      std::max({scan_temp_elems(n), scan_temp_elems(hist), scan_temp_elems(NT)}) + 16;
   const int64_t stride = (n + 63) / 64 * 64;
   const bool has_sigma = pts->sigma[0] != nullptr;
   const bool nhb = nh_blocks_enabled(p, pts, bp);
   const int64_t h_blocks = nhb ? (n + kNhBlock - 1) / kNhBlock + NT + 1 : 0;
   const int64_t h_slots = h_blocks * kNhBlock;
-  struct Req {
-    Slot s;
-    size_t bytes;
-  };
-  const Req reqs[] = {
-      {S_EVENTS, sizeof(EventDesc) * E},
-      {S_CHUNKS, sizeof(Chunk) * C},
-      {S_EXT, sizeof(I) * E * D * 2},
-      {S_GEO, sizeof(T) * E * 3 * D},
-      {S_K0, sizeof(uint32_t) * n},
-      {S_V0, sizeof(int32_t) * n},
-      {S_K1, sizeof(uint32_t) * n},
-      {S_V1, sizeof(int32_t) * n},
-      {S_HIST, sizeof(int32_t) * hist},
-      {S_SCAN, sizeof(int32_t) * scan_elems},
-      {S_TOFF, sizeof(int32_t) * (NT + 1)},
-      {S_ICOUNT, sizeof(int32_t) * NT},
-      {S_NITEMS, sizeof(int32_t) * 4},
-      {S_ITEMS, sizeof(Item) * bp.max_items},
-      {S_QPERM, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
-      {S_QPERM_NH, (bp.tuning.group_queries && p->dm != p->dc && !nhb) ? sizeof(int32_t) * n : 0},
-      {S_QFINE, bp.tuning.group_queries ? sizeof(int32_t) * n : 0},
-      {S_QKEY, bp.tuning.group_queries ? sizeof(uint32_t) * n : 0},
-      {S_HX, sizeof(T) * h_slots * D},
-      {S_HKEY, sizeof(uint64_t) * h_slots},
-      {S_HIDX, sizeof(int32_t) * h_slots},
-      {S_HBOX, sizeof(T) * h_blocks * 2 * D},
-      {S_HBKEY, sizeof(uint64_t) * h_blocks},
-      {S_SRHO, sizeof(T) * n},
-      {S_SKEY, sizeof(uint64_t) * n},
-      {S_SFW, p->kernel == CLUE_KERNEL_FLAT ? sizeof(T) * n : 0},
-      {S_SNH, sizeof(int32_t) * n},
-      {S_SDELTA, sizeof(T) * n},
-      {S_FLAG, sizeof(int32_t) * n},
-      {S_RANK, sizeof(int32_t) * n},
-      {S_NSEEDS, sizeof(int32_t) * 4},
-      {S_SEEDS, sizeof(int32_t) * n},
-      {S_SX, sizeof(T) * stride * D},
-      {S_SW, sizeof(T) * n},
-      {S_SSIG, has_sigma ? sizeof(T) * stride * D : 0},
-      {S_SU, pts->density_uncertainty ? sizeof(T) * n : 0},
-      {S_STAG, sizeof(uint32_t) * n},
-      {S_PERM, sizeof(int32_t) * n},
-      {S_RHO, res->rho ? 0 : sizeof(T) * n},
-      {S_NH, res->nearest_higher ? 0 : sizeof(int32_t) * n},
-      {S_ISSEED, res->is_seed ? 0 : sizeof(int32_t) * n},
-  };
-  for (const Req& r : reqs) CLUE_HIP(ws.buf[r.s].reserve(r.bytes));
+  size_t need[S_NUM];
+  workspace_requests(workspace_shape(bp, p, pts, res), need);
+  for (int k = 0; k < S_NUM; ++k) CLUE_HIP(ws.buf[k].reserve(need[k]));
+
   CLUE_HIP(hipMemcpyAsync(ws.buf[S_EVENTS].p, bp.events.data(), sizeof(EventDesc) * E,
                           hipMemcpyHostToDevice, s));
   CLUE_HIP(hipMemcpyAsync(ws.buf[S_CHUNKS].p, bp.chunks.data(), sizeof(Chunk) * C,
@@ -398,6 +312,10 @@ template clue_status check_points(const clue_params*, const clue_points*, int64_
 template clue_status check_points(const clue_params_f64*, const clue_points_f64*, int64_t);
 template PointArraysT<float> point_arrays(const clue_params*, const clue_points*);
 template PointArraysT<double> point_arrays(const clue_params_f64*, const clue_points_f64*);
+template WorkspaceShape workspace_shape(const BatchPlan&, const clue_params*, const clue_points*,
+                                       const clue_results*);
+template WorkspaceShape workspace_shape(const BatchPlan&, const clue_params_f64*,
+                                       const clue_points_f64*, const clue_results_f64*);
 template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params*,
                                    const clue_points*, clue_results*, hipStream_t,
                                    BatchBuffersT<float>*, ResultArraysT<float>*);

@@ -7,7 +7,11 @@
 // parameter class into one batch, uploads the batch, runs the batched pipeline
 // with per-event cluster numbering, and copies the labels back. Lanes run
 // concurrently, so batch k+1 is uploaded and computed while batch k finishes.
-// Admission is bounded by the HBM budget through the per-lane point cap.
+// Admission is bounded by the HBM budget: a batch grows only while the device
+// bytes its lane would then hold (the pipeline workspace sized by the same table
+// as every run, plus the lane's staging, columns and labels) stay within the
+// lane's share of the budget. The scheduling state machine is host-only code
+// (clue_serve_core.h); this file supplies the HIP lanes behind it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +27,7 @@
 
 #include "clue_amd.h"
 #include "clue_runtime.h"
+#include "clue_serve_core.h"
 
 using namespace clue_rt;
 
@@ -47,118 +52,132 @@ __global__ void k_repack(const ReqDesc* __restrict__ reqs, const clue_dev::Chunk
   }
 }
 
-int64_t now_ns() {
-  return std::chrono::duration_cast<std::chrono::nanoseconds>(
-             std::chrono::steady_clock::now().time_since_epoch())
-      .count();
+// Points per launch: 32-bit indexing of the batched pipeline, and the labels
+// buffer each lane pins on the host.
+constexpr int64_t kMaxLanePoints = int64_t(16) << 20;
+
+// Device bytes of one lane for a batch of this shape: the pipeline workspace (the
+// request table prepare_batch reserves from) plus the lane's own staging,
+// columns, labels and descriptor buffers, each as its grow-only buffer would hold
+// it. `caps` (NULL: nothing held yet) are the lane's current capacities.
+int64_t lane_bytes(const clue_params& p, const clue_srv::BatchShape& shape, const size_t* ws_caps,
+                   const size_t* own_caps) {
+  BatchPlan bp;
+  bp.n_total = shape.points;
+  bp.n_tiles_total = shape.tiles;
+  bp.max_items = (shape.points + 63) / 64 + shape.tiles;
+  bp.tuning = table_tuning(p.ndim, shape.points, shape.events);
+  bp.events.resize(static_cast<size_t>(shape.events));
+  bp.chunks.resize(static_cast<size_t>(shape.chunks));
+  clue_points pts{};
+  clue_results res{};
+  int32_t dummy = 0;
+  res.cluster_index = &dummy;  // the lane's labels buffer; rho, nh, is_seed in the workspace
+  size_t need[S_NUM];
+  workspace_requests(workspace_shape(bp, &p, &pts, &res), need);
+  int64_t total = 0;
+  for (int k = 0; k < S_NUM; ++k) total += grown_capacity(ws_caps ? ws_caps[k] : 0, need[k]);
+  const size_t col_bytes = sizeof(float) * static_cast<size_t>(shape.points) * (p.ndim + 1);
+  const size_t own[4] = {col_bytes, col_bytes, sizeof(int32_t) * static_cast<size_t>(shape.points),
+                         sizeof(ReqDesc) * static_cast<size_t>(shape.events)};
+  for (int k = 0; k < 4; ++k) total += grown_capacity(own_caps ? own_caps[k] : 0, own[k]);
+  return total;
 }
 
-// Device bytes per point of one lane (staging, columns, labels and the batch
-// workspace), an upper estimate used for the HBM budget.
-constexpr int64_t kLaneBytesPerPoint = 192;
-
-bool same_class(const clue_params& a, const clue_params& b) {
-  return std::memcmp(&a, &b, sizeof(clue_params)) == 0;
-}
-
-}  // namespace
-
-struct clue_server {
-  struct Request {
-    uint64_t id = 0;
-    clue_params params{};
-    int32_t n = 0;
-    size_t ring_off = 0;
-    size_t ring_bytes = 0;
-    int64_t submit_ns = 0, launch_ns = 0, done_ns = 0;
-    int32_t batch_events = 0;
-    int64_t batch_points = 0;
-    std::vector<int32_t> labels;
-    bool done = false;
-    bool ring_released = false;
-    clue_status status = CLUE_OK;
-    std::string error;
-  };
-  using RequestP = std::shared_ptr<Request>;
-
+// The lanes on HIP streams. Completion of a launch is signalled by a host
+// function queued after its label download; the worker then reads the lane's
+// event to tell success from an asynchronous failure.
+class HipLanes : public clue_srv::ServeBackend {
+ public:
   struct Lane {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     Workspace ws;
     BatchPlan plan;
     DevBuf staging, cols, labels, reqdesc;
-    int32_t* labels_host = nullptr;  // pinned, lane capacity
+    int32_t* labels_host = nullptr;   // pinned, lane point cap
     ReqDesc* reqdesc_host = nullptr;  // pinned
     size_t reqdesc_cap = 0;
-    std::vector<RequestP> reqs;
     std::vector<int32_t> offsets;
-    bool busy = false;
+    HipLanes* owner = nullptr;
   };
 
-  clue_server_config cfg{};
-  int64_t lane_points = 0;
-  std::vector<std::unique_ptr<Lane>> lanes;
+  explicit HipLanes(int device) : device_(device) {}
+  ~HipLanes() override {
+    ScopedDevice dev(device_);
+    for (auto& L : lanes_) {
+      if (L->stream) (void)hipStreamSynchronize(L->stream);
+      if (L->labels_host) (void)hipHostFree(L->labels_host);
+      if (L->reqdesc_host) (void)hipHostFree(L->reqdesc_host);
+      if (L->done) (void)hipEventDestroy(L->done);
+      if (L->stream) (void)hipStreamDestroy(L->stream);
+    }
+  }
+  clue_status add_lane(int64_t lane_points) {
+    auto L = std::make_unique<Lane>();
+    L->owner = this;
+    CLUE_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    CLUE_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+    CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&L->labels_host),
+                           sizeof(int32_t) * static_cast<size_t>(lane_points), hipHostMallocDefault));
+    lanes_.push_back(std::move(L));
+    return CLUE_OK;
+  }
+  void set_core(clue_srv::ServeCore* core) { core_ = core; }
 
-  // Pinned staging ring (FIFO allocation, released in order of submission).
-  char* ring = nullptr;
-  size_t ring_cap = 0;
-  std::deque<RequestP> ring_live;
+  int lanes() const override { return static_cast<int>(lanes_.size()); }
 
-  std::deque<RequestP> waiting;
-  std::unordered_map<uint64_t, RequestP> table;
-  std::mutex mu;
-  std::condition_variable cv_done, cv_space, cv_work;
-  std::thread worker;
-  bool stop = false;
-  uint64_t next_id = 1;
-  clue_server_stats stats{};
-  std::string fatal;
-
-  // Worker-side failure: recorded and reported to every later call.
-  void set_fatal(const std::string& msg) {
-    std::lock_guard<std::mutex> g(mu);
-    if (fatal.empty()) fatal = msg;
-    cv_done.notify_all();
-    cv_space.notify_all();
+  clue_status launch(int lane, const std::vector<clue_srv::RequestP>& batch, const char* ring,
+                     std::string* err) override {
+    const clue_status st = launch_impl(*lanes_[lane], batch, ring);
+    if (st) *err = last_error();
+    return st;
   }
 
-  // Ring allocation; caller holds mu. Returns false when there is no room now.
-  // Live blocks are contiguous in ring order from the oldest (tail) to the newest;
-  // the live region has wrapped when the oldest block sits above the newest.
-  bool ring_alloc(size_t bytes, size_t* off) {
-    if (bytes > ring_cap) return false;
-    if (ring_live.empty()) {
-      *off = 0;
-      return true;
-    }
-    const size_t tail = ring_live.front()->ring_off;
-    const size_t head = ring_live.back()->ring_off + ring_live.back()->ring_bytes;
-    const bool wrapped = ring_live.front()->ring_off > ring_live.back()->ring_off;
-    if (!wrapped) {
-      if (ring_cap - head >= bytes) {
-        *off = head;
-        return true;
-      }
-      if (tail >= bytes) {
-        *off = 0;
-        return true;
-      }
-      return false;
-    }
-    if (tail - head >= bytes) {
-      *off = head;
-      return true;
-    }
-    return false;
+  clue_srv::LanePoll poll(int lane, std::string* err) override {
+    const hipError_t e = hipEventQuery(lanes_[lane]->done);
+    if (e == hipSuccess) return clue_srv::LanePoll::kDone;
+    if (e == hipErrorNotReady) return clue_srv::LanePoll::kPending;
+    *err = hipGetErrorString(e);
+    return clue_srv::LanePoll::kFailed;
   }
 
-  // Caller holds mu: releases the ring blocks of finished requests in FIFO order.
-  void ring_release() {
-    while (!ring_live.empty() && ring_live.front()->ring_released) ring_live.pop_front();
-    cv_space.notify_all();
+  void collect(int lane, const std::vector<clue_srv::RequestP>& batch) override {
+    Lane& L = *lanes_[lane];
+    for (size_t k = 0; k < batch.size(); ++k) {
+      const int32_t* src = L.labels_host + L.offsets[k];
+      batch[k]->labels.assign(src, src + batch[k]->n);
+    }
   }
 
-  clue_status launch(Lane& L, std::vector<RequestP>& batch) {
+  void drain(int lane) override { (void)hipStreamSynchronize(lanes_[lane]->stream); }
+
+  int64_t bytes_after(int lane, const clue_params& p,
+                      const clue_srv::BatchShape& shape) const override {
+    const Lane& L = *lanes_[lane];
+    size_t ws_caps[S_NUM];
+    for (int k = 0; k < S_NUM; ++k) ws_caps[k] = L.ws.buf[k].cap;
+    const size_t own_caps[4] = {L.staging.cap, L.cols.cap, L.labels.cap, L.reqdesc.cap};
+    return lane_bytes(p, shape, ws_caps, own_caps);
+  }
+  int64_t bytes_fresh(const clue_params& p, const clue_srv::BatchShape& shape) const override {
+    return lane_bytes(p, shape, nullptr, nullptr);
+  }
+  void trim(int lane) override {
+    Lane& L = *lanes_[lane];
+    ScopedDevice dev(device_);
+    for (DevBuf& b : L.ws.buf) b.release();
+    L.staging.release();
+    L.cols.release();
+    L.labels.release();
+    L.reqdesc.release();
+  }
+
+ private:
+  static void on_done(void* lane) { static_cast<Lane*>(lane)->owner->core_->notify(); }
+
+  clue_status launch_impl(Lane& L, const std::vector<clue_srv::RequestP>& batch, const char* ring) {
+    ScopedDevice dev(device_);
     const clue_params& p = batch.front()->params;
     const int D = p.ndim;
     std::vector<uint32_t> sizes(batch.size());
@@ -183,12 +202,12 @@ struct clue_server {
       CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.reqdesc_host),
                              sizeof(ReqDesc) * L.reqdesc_cap, hipHostMallocDefault));
     }
-    // Events with n > 0 are the batch's events, in order (make_batch_plan skips none
-    // here: every submitted event has n >= 1).
+    // Every submitted event has n >= 1, so the plan describes the batch's events
+    // in order.
     float* staging = L.staging.as<float>();
     int64_t src = 0;
     for (size_t k = 0; k < batch.size(); ++k) {
-      const Request& r = *batch[k];
+      const clue_srv::Request& r = *batch[k];
       const size_t bytes = sizeof(float) * static_cast<size_t>(r.n) * (D + 1);
       CLUE_HIP(hipMemcpyAsync(staging + src, ring + r.ring_off, bytes, hipMemcpyHostToDevice,
                               L.stream));
@@ -217,108 +236,28 @@ struct clue_server {
     CLUE_HIP(hipMemcpyAsync(L.labels_host, L.labels.p, sizeof(int32_t) * n_total,
                             hipMemcpyDeviceToHost, L.stream));
     CLUE_HIP(hipEventRecord(L.done, L.stream));
-    const int64_t t = now_ns();
-    for (auto& r : batch) {
-      r->launch_ns = t;
-      r->batch_events = static_cast<int32_t>(batch.size());
-      r->batch_points = n_total;
-    }
-    L.reqs = std::move(batch);
-    L.busy = true;
+    CLUE_HIP(hipLaunchHostFunc(L.stream, &HipLanes::on_done, &L));
     return CLUE_OK;
   }
 
-  void finish(Lane& L) {
-    const int64_t t = now_ns();
-    for (size_t k = 0; k < L.reqs.size(); ++k) {
-      Request& r = *L.reqs[k];
-      r.labels.assign(L.labels_host + L.offsets[k], L.labels_host + L.offsets[k] + r.n);
-    }
-    {
-      std::lock_guard<std::mutex> g(mu);
-      for (auto& r : L.reqs) {
-        r->done_ns = t;
-        r->done = true;
-        r->ring_released = true;
-        stats.events_done += 1;
-        stats.points_done += r->n;
-      }
-      stats.batches += 1;
-      ring_release();
-    }
-    cv_done.notify_all();
-    L.reqs.clear();
-    L.busy = false;
-  }
+  int device_;
+  std::vector<std::unique_ptr<Lane>> lanes_;
+  clue_srv::ServeCore* core_ = nullptr;
+};
 
-  // Takes the waiting events of the oldest event's class, up to the lane cap.
-  std::vector<RequestP> form_batch() {
-    std::vector<RequestP> out;
-    if (waiting.empty()) return out;
-    const clue_params cls = waiting.front()->params;
-    int64_t pts = 0;
-    for (auto it = waiting.begin(); it != waiting.end();) {
-      RequestP r = *it;
-      if (same_class(r->params, cls) && pts + r->n <= lane_points &&
-          static_cast<int64_t>(out.size()) < cfg.max_batch_events) {
-        pts += r->n;
-        out.push_back(r);
-        it = waiting.erase(it);
-      } else {
-        ++it;
-        if (pts >= lane_points) break;
-      }
-    }
-    return out;
-  }
+}  // namespace
 
-  void run() {
-    (void)hipSetDevice(cfg.device);
-    while (true) {
-      bool progressed = false;
-      for (auto& L : lanes) {
-        if (L->busy && hipEventQuery(L->done) == hipSuccess) {
-          finish(*L);
-          progressed = true;
-        }
-      }
-      for (auto& L : lanes) {
-        if (L->busy) continue;
-        std::vector<RequestP> batch;
-        {
-          std::lock_guard<std::mutex> g(mu);
-          batch = form_batch();
-        }
-        if (batch.empty()) break;
-        std::vector<RequestP> keep = batch;
-        const clue_status st = launch(*L, batch);
-        if (st) {
-          std::lock_guard<std::mutex> g(mu);
-          for (auto& r : keep) {
-            r->status = st;
-            r->error = clue_rt::last_error();
-            r->done = true;
-            r->done_ns = now_ns();
-            r->ring_released = true;
-          }
-          ring_release();
-          cv_done.notify_all();
-          L->busy = false;
-          L->reqs.clear();
-        }
-        progressed = true;
-      }
-      std::unique_lock<std::mutex> lk(mu);
-      bool any_busy = false;
-      for (auto& L : lanes) any_busy = any_busy || L->busy;
-      if (stop && waiting.empty() && !any_busy) break;
-      if (!progressed) {
-        if (any_busy)
-          cv_work.wait_for(lk, std::chrono::microseconds(20));
-        else
-          cv_work.wait_for(lk, std::chrono::milliseconds(5),
-                           [&] { return stop || !waiting.empty(); });
-      }
+struct clue_server {
+  clue_server_config cfg{};
+  HipLanes lanes;
+  char* ring = nullptr;
+  std::unique_ptr<clue_srv::ServeCore> core;
+  explicit clue_server(int device) : lanes(device) {}
+  ~clue_server() {
+    core.reset();  // finishes queued work and joins the worker
+    if (ring) {
+      ScopedDevice dev(cfg.device);
+      (void)hipHostFree(ring);
     }
   }
 };
@@ -331,54 +270,33 @@ clue_status clue_server_create(const clue_server_config* cfg, clue_server** out)
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || cfg->device < 0 || cfg->device >= count)
     return fail(CLUE_ERR_NO_DEVICE, "no such HIP device");
-  std::unique_ptr<clue_server> s(new clue_server());
+  std::unique_ptr<clue_server> s(new clue_server(cfg->device));
   s->cfg = *cfg;
   if (s->cfg.n_lanes <= 0) s->cfg.n_lanes = 2;
   if (s->cfg.hbm_budget_bytes <= 0) s->cfg.hbm_budget_bytes = int64_t(64) << 30;
   if (s->cfg.staging_bytes <= 0) s->cfg.staging_bytes = int64_t(1) << 30;
   if (s->cfg.max_batch_events <= 0) s->cfg.max_batch_events = 1 << 20;
-  const int64_t by_budget = s->cfg.hbm_budget_bytes / (s->cfg.n_lanes * kLaneBytesPerPoint);
-  s->lane_points = s->cfg.max_batch_points > 0 ? std::min(s->cfg.max_batch_points, by_budget)
-                                               : std::min<int64_t>(by_budget, 16 << 20);
-  if (s->lane_points < 1) return fail(CLUE_ERR_CAPACITY, "HBM budget too small for one point");
+  clue_srv::ServeCore::Limits lim;
+  lim.lane_budget_bytes = s->cfg.hbm_budget_bytes / s->cfg.n_lanes;
+  lim.lane_points = s->cfg.max_batch_points > 0 ? std::min<int64_t>(s->cfg.max_batch_points, kMaxLanePoints)
+                                                : kMaxLanePoints;
+  lim.max_batch_events = s->cfg.max_batch_events;
+  lim.staging_bytes = static_cast<size_t>(s->cfg.staging_bytes);
   ScopedDevice dev(cfg->device);
   for (int k = 0; k < s->cfg.n_lanes; ++k) {
-    auto L = std::make_unique<clue_server::Lane>();
-    CLUE_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
-    CLUE_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
-    CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&L->labels_host),
-                           sizeof(int32_t) * static_cast<size_t>(s->lane_points),
-                           hipHostMallocDefault));
-    s->lanes.push_back(std::move(L));
+    const clue_status st = s->lanes.add_lane(lim.lane_points);
+    if (st) return st;
   }
-  s->ring_cap = static_cast<size_t>(s->cfg.staging_bytes);
-  CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->ring), s->ring_cap, hipHostMallocDefault));
-  s->worker = std::thread([srv = s.get()] { srv->run(); });
+  CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->ring), lim.staging_bytes, hipHostMallocDefault));
+  s->core = std::make_unique<clue_srv::ServeCore>(&s->lanes, s->ring, lim);
+  s->lanes.set_core(s->core.get());
+  s->core->start();
   *out = s.release();
   return CLUE_OK;
 }
 
 clue_status clue_server_destroy(clue_server* s) {
-  if (!s) return CLUE_OK;
-  {
-    std::lock_guard<std::mutex> g(s->mu);
-    s->stop = true;
-  }
-  s->cv_work.notify_all();
-  if (s->worker.joinable()) s->worker.join();
-  {
-    ScopedDevice dev(s->cfg.device);
-    for (auto& L : s->lanes) {
-      (void)hipStreamSynchronize(L->stream);
-      if (L->labels_host) (void)hipHostFree(L->labels_host);
-      if (L->reqdesc_host) (void)hipHostFree(L->reqdesc_host);
-      (void)hipEventDestroy(L->done);
-      (void)hipStreamDestroy(L->stream);
-    }
-    s->lanes.clear();
-    if (s->ring) (void)hipHostFree(s->ring);
-  }
-  delete s;
+  delete s;  // finishes queued work first
   return CLUE_OK;
 }
 
@@ -390,72 +308,25 @@ clue_status clue_server_submit(clue_server* s, const clue_params* params, int32_
   if (params->metric == CLUE_METRIC_MAHALANOBIS)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "served events carry no sigma (Mahalanobis)");
   if (n < 1) return fail(CLUE_ERR_INVALID_ARGUMENT, "the number of points must be positive");
-  if (n > s->lane_points)
-    return fail(CLUE_ERR_CAPACITY, "event larger than the per-launch point cap of the HBM budget");
-  const size_t bytes = (sizeof(float) * static_cast<size_t>(n) * (params->ndim + 1) + 255) & ~size_t(255);
-  auto r = std::make_shared<clue_server::Request>();
-  r->params = *params;
-  r->n = n;
-  r->ring_bytes = bytes;
-  std::unique_lock<std::mutex> lk(s->mu);
-  if (!s->fatal.empty()) return fail(CLUE_ERR_RUNTIME, s->fatal);
-  if (bytes > s->ring_cap) return fail(CLUE_ERR_CAPACITY, "event larger than the staging ring");
-  size_t off = 0;
-  s->cv_space.wait(lk, [&] { return s->ring_alloc(bytes, &off) || !s->fatal.empty(); });
-  if (!s->fatal.empty()) return fail(CLUE_ERR_RUNTIME, s->fatal);
-  r->ring_off = off;
-  r->id = s->next_id++;
-  s->ring_live.push_back(r);
-  lk.unlock();
-  // Copy outside the lock: the block is reserved for this request.
-  std::memcpy(s->ring + off, host_data, sizeof(float) * static_cast<size_t>(n) * (params->ndim + 1));
-  r->submit_ns = now_ns();
-  lk.lock();
-  s->table[r->id] = r;
-  s->waiting.push_back(r);
-  s->stats.events_submitted += 1;
-  lk.unlock();
-  s->cv_work.notify_one();
-  *id = r->id;
-  return CLUE_OK;
+  std::string err;
+  st = s->core->submit(*params, n, host_data, id, &err);
+  return st ? fail(st, err) : CLUE_OK;
 }
 
 clue_status clue_server_wait(clue_server* s, uint64_t id, int32_t* host_labels,
                              int64_t timeout_us, clue_server_timing* timing) {
   if (!s) return fail(CLUE_ERR_INVALID_ARGUMENT, "null server");
-  std::unique_lock<std::mutex> lk(s->mu);
-  auto it = s->table.find(id);
-  if (it == s->table.end()) return fail(CLUE_ERR_INVALID_ARGUMENT, "unknown request id");
-  clue_server::RequestP r = it->second;
-  auto ready = [&] { return r->done || !s->fatal.empty(); };
-  if (timeout_us < 0) {
-    s->cv_done.wait(lk, ready);
-  } else if (!s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), ready)) {
-    return fail(CLUE_ERR_RUNTIME, "timeout");
-  }
-  if (!r->done) return fail(CLUE_ERR_RUNTIME, s->fatal);
-  s->table.erase(id);
-  lk.unlock();
-  if (r->status) return fail(r->status, r->error);
-  if (host_labels) std::memcpy(host_labels, r->labels.data(), sizeof(int32_t) * r->labels.size());
-  if (timing) {
-    timing->submit_ns = r->submit_ns;
-    timing->launch_ns = r->launch_ns;
-    timing->done_ns = r->done_ns;
-    timing->batch_events = r->batch_events;
-    timing->batch_points = r->batch_points;
-  }
-  return CLUE_OK;
+  std::string err;
+  const clue_status st = s->core->wait(id, host_labels, timeout_us, timing, &err);
+  return st ? fail(st, err) : CLUE_OK;
 }
 
 clue_status clue_server_get_stats(clue_server* s, clue_server_stats* out) {
   if (!s || !out) return fail(CLUE_ERR_INVALID_ARGUMENT, "null argument");
-  std::lock_guard<std::mutex> g(s->mu);
-  *out = s->stats;
-  out->lane_points = s->lane_points;
+  s->core->get_stats(out);
   return CLUE_OK;
 }
 
-int64_t clue_now_ns(void) { return now_ns(); }
+int64_t clue_now_ns(void) { return clue_srv::now_ns(); }
 
 }  // extern "C"

@@ -10,6 +10,16 @@ constexpr int kMaxDim = 10;
 // one largest key per block).
 constexpr int kNhBlock = 16;
 
+// Workspace sizes of the scan and radix-sort primitives (clue_primitives.hip),
+// here so that host-only code sizes workspaces exactly as the device path does.
+constexpr int kScanTileElems = 4096;  // elements per block of the exclusive scan
+constexpr int kRsTileKeys = 4096;     // keys per block of a radix pass
+constexpr int kRsDigitBins = 256;     // bins of one radix digit
+inline int64_t scan_temp_elems(int64_t n) { return (n + kScanTileElems - 1) / kScanTileElems + 1; }
+inline int64_t radix_hist_elems(int64_t n) {
+  return kRsDigitBins * ((n + kRsTileKeys - 1) / kRsTileKeys);
+}
+
 // One event of a batch. All offsets are host-computed from the event sizes.
 struct EventDesc {
   int32_t p_off;   // first point of the event in the packed arrays

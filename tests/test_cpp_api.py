@@ -1,6 +1,7 @@
-"""Runs the C++ drop-in test programs (tests/cpp/*.cpp, built by
-__graft_entry__.build()): the metric call operators and host point forms on the
-CPU, the clustering cases on the GPU."""
+"""Runs the C++ test programs (tests/cpp/*.cpp, built by __graft_entry__.build()):
+the metric call operators and host point forms, and the host-only runtime and the
+oracle under the address and undefined-behaviour sanitizers, on the CPU; the
+clustering cases of the drop-in on the GPU."""
 import os
 import subprocess
 
@@ -21,6 +22,19 @@ def _run(name, *args):
 
 def test_cpp_metrics_and_host_points():
     _run("test_metrics")
+
+
+def test_host_runtime_under_sanitizers():
+    """Batch plans, workspace sizing, the staging ring and the server state machine
+    (scripted device: normal runs, budget splits, failed launches, asynchronous
+    device failures, back-pressure), built with ASan and UBSan."""
+    _run("test_host_runtime")
+
+
+def test_oracle_under_sanitizers():
+    """Every oracle entry point over kernels, metrics, optional inputs, precisions and
+    batched geometries, built with ASan and UBSan."""
+    _run("test_oracle_sanitized")
 
 
 @pytest.mark.gpu

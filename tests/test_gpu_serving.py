@@ -24,8 +24,10 @@ def _alone(ctx, ev):
 
 
 def test_served_equals_alone_in_the_mix(gpu_ctx, oracle_mod):
-    pool = make_pool(40, seed=11, lo=64, hi=20000)
-    with Server(n_lanes=2, max_batch_points=60000) as srv:
+    # Sizes up to the served stream's largest (262144 points), 2-D and 3-D mixed.
+    pool = make_pool(40, seed=11, lo=64, hi=262144)
+    assert max(ev.n for ev in pool) > 100000
+    with Server(n_lanes=2, max_batch_points=600000) as srv:
         ids = [srv.submit(ev.params, ev.data) for ev in pool]      # all at once: a mix
         results = [srv.wait(i) for i in ids]
         st = srv.stats()
@@ -69,3 +71,21 @@ def test_server_rejects_oversized_and_bad_events():
         rid = srv.submit(p, np.random.default_rng(0).uniform(0, 5, (3, 500)).astype(np.float32))
         labels, t = srv.wait(rid)
         assert labels.shape == (500,) and t["done_ns"] >= t["launch_ns"] >= t["submit_ns"]
+
+
+def test_batches_respect_the_hbm_budget():
+    """A lane's batch grows only while the device bytes it would hold (workspace,
+    staging, columns, labels) stay within its share of the budget: with 48 MiB over
+    2 lanes, no batch may exceed 24 MiB at the workspace's smallest per-point size
+    (over 100 bytes a point in 2-D), and every event still completes."""
+    pool = [ev for ev in make_pool(60, seed=13, lo=2000, hi=60000) if ev.ndim == 2][:20]
+    budget = 48 << 20
+    with Server(n_lanes=2, hbm_budget_bytes=budget) as srv:
+        ids = [srv.submit(ev.params, ev.data) for ev in pool]
+        results = [srv.wait(i) for i in ids]
+        # An event that cannot fit one lane's share is refused at submission.
+        huge = make_pool(1, seed=14, lo=250000, hi=262144)[0]
+        with pytest.raises(_capi.ClueError):
+            srv.submit(huge.params, huge.data)
+    assert all(t["batch_points"] * 100 <= budget // 2 for _, t in results)
+    assert max(t["batch_events"] for _, t in results) >= 1
