@@ -155,6 +155,8 @@ _SIGNATURES = {
                                        C.POINTER(Results)]),
     "clue_make_clusters_async": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.POINTER(Points),
                                              C.POINTER(Results)]),
+    "clue_make_clusters_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.POINTER(Points),
+                                               C.POINTER(Results)]),
     "clue_make_clusters_batched": (C.c_int32, [C.c_void_p, C.POINTER(Params), C.POINTER(Points),
                                                C.POINTER(C.c_uint32), C.c_int32,
                                                C.POINTER(Results), C.c_void_p,

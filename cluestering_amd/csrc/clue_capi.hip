@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,72 @@ using namespace clue_dev;
 
 namespace {
 using namespace clue_rt;
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free entry point (clue_make_clusters_noalloc):
+// one array per GPU the library runs on, held until the process ends. It holds
+// the pipeline workspace of one event; kNoAllocScratchBytes covers about a
+// million points in 2-D or 3-D. In host code `scratch` names a host-side stand-in:
+// its device address comes from hipGetSymbolAddress, taken once per device when
+// the first context on that device is created, and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(160) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+constexpr size_t kScratchAlign = 256;
+
+struct ScratchTable {
+  std::mutex mu;
+  bool done[kScratchDevices] = {};
+  hipError_t err[kScratchDevices] = {};
+  void* ptr[kScratchDevices] = {};
+};
+ScratchTable& scratch_table() {
+  static ScratchTable t;
+  return t;
+}
+
+// Takes the device address of `scratch` on `device` (current) once; later calls
+// return what the first one found.
+hipError_t init_scratch(int device) {
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (!t.done[device]) {
+    void* ptr = nullptr;
+    t.err[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    t.ptr[device] = t.err[device] == hipSuccess ? ptr : nullptr;
+    t.done[device] = true;
+  }
+  return t.err[device];
+}
+
+void* scratch_ptr(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  ScratchTable& t = scratch_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  return t.done[device] ? t.ptr[device] : nullptr;
+}
+
+// The descriptors of one event of n points at p_off 0 (what the host plan holds),
+// written on the device: chunk c covers [c * kChunk, min(n, (c + 1) * kChunk)).
+__global__ void k_single_event_descriptors(EventDesc ev, EventDesc* __restrict__ events,
+                                           Chunk* __restrict__ chunks, int n_chunks) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c == 0) events[0] = ev;
+  if (c < n_chunks) {
+    Chunk ch{};
+    ch.event = 0;
+    ch.begin = c * kChunk;
+    ch.end = min(ev.n, (c + 1) * kChunk);
+    chunks[c] = ch;
+  }
+}
+
 }  // namespace
 
 struct clue_ctx {
@@ -156,6 +223,12 @@ clue_status clue_context_create(int32_t device, void* stream, clue_ctx** out) {
     CLUE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->own_stream = true;
   }
+  // Set-up of the allocation-free entry point: the scratch address on this device
+  // (its failure is reported by that entry point only), and host descriptors with
+  // room for any event the scratch can hold, so that planning it allocates nothing.
+  (void)init_scratch(device);
+  c->plan.events.reserve(1);
+  c->plan.chunks.reserve(kNoAllocScratchBytes / kChunk / 64 + 1);
   *out = c.release();
   return CLUE_OK;
 }
@@ -212,6 +285,69 @@ clue_status clue_make_clusters_batched(clue_ctx* ctx, const clue_params* params,
   clue_status st = run_batch(ctx, params, points, sizes, n_events, results, event_of_cluster);
   if (st) return st;
   if (n_clusters) return clue_get_n_clusters(ctx, n_clusters);
+  return CLUE_OK;
+}
+
+// Allocation-free variant of clue_make_clusters: the pipeline workspace lives in
+// the static scratch of the context's device instead of the context's pooled
+// buffers. No hipMalloc, hipFree, memset or host-to-device copy: the event's
+// descriptors are written by a kernel, and every stage initialises what it reads.
+clue_status clue_make_clusters_noalloc(clue_ctx* ctx, const clue_params* params,
+                                       const clue_points* points, clue_results* results) {
+  if (!ctx) return fail(CLUE_ERR_INVALID_ARGUMENT, "null context");
+  clue_status st = validate(params);
+  if (st) return st;
+  if (!points) return fail(CLUE_ERR_INVALID_ARGUMENT, "null points");
+  if (points->n < 1) return fail(CLUE_ERR_INVALID_ARGUMENT, "the number of points must be positive");
+  if (!results || !results->cluster_index)
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "cluster_index is required");
+  const size_t n_chunks = (static_cast<size_t>(points->n) + kChunk - 1) / kChunk;
+  if (n_chunks > ctx->plan.chunks.capacity())
+    return fail(CLUE_ERR_CAPACITY, "event larger than the allocation-free scratch holds (" +
+                                       std::to_string(kNoAllocScratchBytes >> 20) + " MiB)");
+  ScopedDevice dev(ctx->device);
+  const hipError_t se = init_scratch(ctx->device);
+  if (se != hipSuccess)
+    return fail(CLUE_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se));
+  char* base = static_cast<char*>(scratch_ptr(ctx->device));
+  if (ctx->pending) {
+    CLUE_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->pending = false;
+  }
+  BatchPlan& bp = ctx->plan;
+  const uint32_t n = static_cast<uint32_t>(points->n);
+  st = make_batch_plan(params, &n, 1, &bp, ctx->tuned ? &ctx->tuning : nullptr);
+  if (st) return st;
+  st = check_points(params, points, bp.n_total);
+  if (st) return st;
+  size_t need[S_NUM];
+  workspace_requests(workspace_shape(bp, params, points, results), need);
+  void* slot[S_NUM];
+  size_t off = 0;
+  for (int k = 0; k < S_NUM; ++k) {
+    slot[k] = base + off;
+    off += (need[k] + kScratchAlign - 1) / kScratchAlign * kScratchAlign;
+  }
+  if (off > kNoAllocScratchBytes)
+    return fail(CLUE_ERR_CAPACITY, "event needs " + std::to_string(off) +
+                                       " bytes of scratch; the allocation-free scratch holds " +
+                                       std::to_string(kNoAllocScratchBytes));
+  BatchBuffers bb;
+  ResultArrays ra;
+  st = bind_batch(slot, bp, params, points, results, ctx->stream, &bb, &ra, false);
+  if (st) return st;
+  hipLaunchKernelGGL(k_single_event_descriptors, dim3(static_cast<unsigned>((n_chunks + 255) / 256)),
+                     dim3(256), 0, ctx->stream, bp.events[0], at<EventDesc>(slot, S_EVENTS),
+                     at<Chunk>(slot, S_CHUNKS), static_cast<int>(n_chunks));
+  const RunParams rp = make_run_params(params, points, bp);
+  for (bool& r : ctx->ev_recorded) r = false;
+  launch_pipeline(rp, point_arrays(params, points), bb, ra, ctx->stream, nullptr);
+  CLUE_HIP(hipGetLastError());
+  ctx->has_run = true;
+  ctx->last_n = bp.n_total;
+  ctx->last_seeds = bb.seeds;
+  ctx->last_nseeds = bb.n_seeds;
+  CLUE_HIP(hipStreamSynchronize(ctx->stream));
   return CLUE_OK;
 }
 

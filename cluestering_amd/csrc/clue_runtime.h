@@ -99,6 +99,19 @@ PointArraysT<value_t<P>> point_arrays(const P* p, const Pts* pts);
 // The workspace shape of a batch run (what workspace_requests sizes).
 template <class P, class Pts, class Res>
 WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, const Res* res);
+// Slot k of a workspace as a U array.
+template <class U>
+U* at(void* const (&slot)[S_NUM], int k) {
+  return static_cast<U*>(slot[k]);
+}
+// Fills the BatchBuffers and result pointers of one batch from workspace slots
+// that hold at least workspace_requests' bytes each (NULL results use workspace
+// slots) and, with `upload`, copies its descriptors from the host on stream s
+// (without, the caller writes slot[S_EVENTS] and slot[S_CHUNKS] on the device).
+template <class P, class Pts, class Res>
+clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P* p, const Pts* pts,
+                       Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
+                       ResultArraysT<value_t<P>>* ra, bool upload);
 // Reserves the workspace of one batch, uploads its descriptors on stream s and
 // fills the BatchBuffers and result pointers (NULL results use workspace buffers).
 template <class P, class Pts, class Res>

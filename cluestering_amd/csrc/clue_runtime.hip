@@ -193,12 +193,13 @@ WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, 
   return w;
 }
 
-// Reserves the workspace of one batch and fills the BatchBuffers. Descriptor
-// uploads are enqueued on the stream (they precede any kernel that reads them).
+// Fills the BatchBuffers and result pointers of one batch from workspace slots
+// (each at least workspace_requests' bytes) and, with `upload`, enqueues the
+// descriptor uploads on the stream (they precede any kernel that reads them).
 template <class P, class Pts, class Res>
-clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
-                          Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
-                          ResultArraysT<value_t<P>>* ra) {
+clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P* p, const Pts* pts,
+                       Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
+                       ResultArraysT<value_t<P>>* ra, bool upload) {
   using T = value_t<P>;
   using I = typename FloatBits<T>::Int;
   const int D = p->ndim;
@@ -211,14 +212,12 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
   const bool nhb = nh_blocks_enabled(p, pts, bp);
   const int64_t h_blocks = nhb ? (n + kNhBlock - 1) / kNhBlock + NT + 1 : 0;
   const int64_t h_slots = h_blocks * kNhBlock;
-  size_t need[S_NUM];
-  workspace_requests(workspace_shape(bp, p, pts, res), need);
-  for (int k = 0; k < S_NUM; ++k) CLUE_HIP(ws.buf[k].reserve(need[k]));
-
-  CLUE_HIP(hipMemcpyAsync(ws.buf[S_EVENTS].p, bp.events.data(), sizeof(EventDesc) * E,
-                          hipMemcpyHostToDevice, s));
-  CLUE_HIP(hipMemcpyAsync(ws.buf[S_CHUNKS].p, bp.chunks.data(), sizeof(Chunk) * C,
-                          hipMemcpyHostToDevice, s));
+  if (upload) {
+    CLUE_HIP(hipMemcpyAsync(slot[S_EVENTS], bp.events.data(), sizeof(EventDesc) * E,
+                            hipMemcpyHostToDevice, s));
+    CLUE_HIP(hipMemcpyAsync(slot[S_CHUNKS], bp.chunks.data(), sizeof(Chunk) * C,
+                            hipMemcpyHostToDevice, s));
+  }
   auto& b = *bb;
   b = BatchBuffersT<T>{};
   b.n_events = static_cast<int32_t>(E);
@@ -227,59 +226,73 @@ clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const 
   b.n_tiles_total = static_cast<int32_t>(NT);
   b.max_items = static_cast<int32_t>(bp.max_items);
   b.key_bits = bp.key_bits;
-  b.events = ws.buf[S_EVENTS].as<EventDesc>();
-  b.chunks = ws.buf[S_CHUNKS].as<Chunk>();
-  b.ext = ws.buf[S_EXT].as<I>();
-  b.geo = ws.buf[S_GEO].as<T>();
-  b.keys0 = ws.buf[S_K0].as<uint32_t>();
-  b.vals0 = ws.buf[S_V0].as<int32_t>();
-  b.keys1 = ws.buf[S_K1].as<uint32_t>();
-  b.vals1 = ws.buf[S_V1].as<int32_t>();
-  b.rs_hist = ws.buf[S_HIST].as<int32_t>();
-  b.scan_tmp = ws.buf[S_SCAN].as<int32_t>();
-  b.toff = ws.buf[S_TOFF].as<int32_t>();
-  b.icount = ws.buf[S_ICOUNT].as<int32_t>();
-  b.n_items = ws.buf[S_NITEMS].as<int32_t>();
-  b.items = ws.buf[S_ITEMS].as<Item>();
-  b.qperm = bp.tuning.group_queries ? ws.buf[S_QPERM].as<int32_t>() : nullptr;
+  b.events = at<EventDesc>(slot, S_EVENTS);
+  b.chunks = at<Chunk>(slot, S_CHUNKS);
+  b.ext = at<I>(slot, S_EXT);
+  b.geo = at<T>(slot, S_GEO);
+  b.keys0 = at<uint32_t>(slot, S_K0);
+  b.vals0 = at<int32_t>(slot, S_V0);
+  b.keys1 = at<uint32_t>(slot, S_K1);
+  b.vals1 = at<int32_t>(slot, S_V1);
+  b.rs_hist = at<int32_t>(slot, S_HIST);
+  b.scan_tmp = at<int32_t>(slot, S_SCAN);
+  b.toff = at<int32_t>(slot, S_TOFF);
+  b.icount = at<int32_t>(slot, S_ICOUNT);
+  b.n_items = at<int32_t>(slot, S_NITEMS);
+  b.items = at<Item>(slot, S_ITEMS);
+  b.qperm = bp.tuning.group_queries ? at<int32_t>(slot, S_QPERM) : nullptr;
   // The block scan orders its queries by qfine: no separate order for dm.
   b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc && !nhb)
-                   ? ws.buf[S_QPERM_NH].as<int32_t>()
+                   ? at<int32_t>(slot, S_QPERM_NH)
                    : b.qperm;
-  b.qfine = bp.tuning.group_queries ? ws.buf[S_QFINE].as<int32_t>() : nullptr;
-  b.qkey = bp.tuning.group_queries ? ws.buf[S_QKEY].as<uint32_t>() : nullptr;
-  b.srho = ws.buf[S_SRHO].as<T>();
-  b.skey = ws.buf[S_SKEY].as<uint64_t>();
-  b.snh = ws.buf[S_SNH].as<int32_t>();
-  b.sdelta = ws.buf[S_SDELTA].as<T>();
-  b.flag = ws.buf[S_FLAG].as<int32_t>();
-  b.rank = ws.buf[S_RANK].as<int32_t>();
-  b.n_seeds = ws.buf[S_NSEEDS].as<int32_t>();
-  b.seeds = ws.buf[S_SEEDS].as<int32_t>();
+  b.qfine = bp.tuning.group_queries ? at<int32_t>(slot, S_QFINE) : nullptr;
+  b.qkey = bp.tuning.group_queries ? at<uint32_t>(slot, S_QKEY) : nullptr;
+  b.srho = at<T>(slot, S_SRHO);
+  b.skey = at<uint64_t>(slot, S_SKEY);
+  b.snh = at<int32_t>(slot, S_SNH);
+  b.sdelta = at<T>(slot, S_SDELTA);
+  b.flag = at<int32_t>(slot, S_FLAG);
+  b.rank = at<int32_t>(slot, S_RANK);
+  b.n_seeds = at<int32_t>(slot, S_NSEEDS);
+  b.seeds = at<int32_t>(slot, S_SEEDS);
   b.sorted_stride = stride;
-  b.hx = nhb ? ws.buf[S_HX].as<T>() : nullptr;
-  b.hkey = nhb ? ws.buf[S_HKEY].as<uint64_t>() : nullptr;
-  b.hidx = nhb ? ws.buf[S_HIDX].as<int32_t>() : nullptr;
-  b.hbox = nhb ? ws.buf[S_HBOX].as<T>() : nullptr;
-  b.hbkey = nhb ? ws.buf[S_HBKEY].as<uint64_t>() : nullptr;
+  b.hx = nhb ? at<T>(slot, S_HX) : nullptr;
+  b.hkey = nhb ? at<uint64_t>(slot, S_HKEY) : nullptr;
+  b.hidx = nhb ? at<int32_t>(slot, S_HIDX) : nullptr;
+  b.hbox = nhb ? at<T>(slot, S_HBOX) : nullptr;
+  b.hbkey = nhb ? at<uint64_t>(slot, S_HBKEY) : nullptr;
   b.h_stride = h_slots;
   for (int d = 0; d < kMaxDim; ++d) {
-    b.sorted.x[d] = d < D ? ws.buf[S_SX].as<T>() + stride * d : nullptr;
-    b.sorted.sigma[d] = (has_sigma && d < D) ? ws.buf[S_SSIG].as<T>() + stride * d : nullptr;
+    b.sorted.x[d] = d < D ? at<T>(slot, S_SX) + stride * d : nullptr;
+    b.sorted.sigma[d] = (has_sigma && d < D) ? at<T>(slot, S_SSIG) + stride * d : nullptr;
   }
-  b.sorted.w = ws.buf[S_SW].as<T>();
-  b.sorted.fw = p->kernel == CLUE_KERNEL_FLAT ? ws.buf[S_SFW].as<T>() : nullptr;
-  b.sorted.u = pts->density_uncertainty ? ws.buf[S_SU].as<T>() : nullptr;
-  b.sorted.tag = ws.buf[S_STAG].as<uint32_t>();
-  b.sorted.perm = ws.buf[S_PERM].as<int32_t>();
+  b.sorted.w = at<T>(slot, S_SW);
+  b.sorted.fw = p->kernel == CLUE_KERNEL_FLAT ? at<T>(slot, S_SFW) : nullptr;
+  b.sorted.u = pts->density_uncertainty ? at<T>(slot, S_SU) : nullptr;
+  b.sorted.tag = at<uint32_t>(slot, S_STAG);
+  b.sorted.perm = at<int32_t>(slot, S_PERM);
   ra->cluster = res->cluster_index;
-  ra->is_seed = res->is_seed ? res->is_seed : ws.buf[S_ISSEED].as<int32_t>();
-  ra->rho = res->rho ? res->rho : ws.buf[S_RHO].as<T>();
-  ra->nh = res->nearest_higher ? res->nearest_higher : ws.buf[S_NH].as<int32_t>();
+  ra->is_seed = res->is_seed ? res->is_seed : at<int32_t>(slot, S_ISSEED);
+  ra->rho = res->rho ? res->rho : at<T>(slot, S_RHO);
+  ra->nh = res->nearest_higher ? res->nearest_higher : at<int32_t>(slot, S_NH);
   ra->delta = res->delta;
   ra->event_of_cluster = nullptr;
   ra->local_ids = 0;
   return CLUE_OK;
+}
+
+
+// Reserves the workspace of one batch in grow-only buffers, then binds it.
+template <class P, class Pts, class Res>
+clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
+                          Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
+                          ResultArraysT<value_t<P>>* ra) {
+  size_t need[S_NUM];
+  workspace_requests(workspace_shape(bp, p, pts, res), need);
+  for (int k = 0; k < S_NUM; ++k) CLUE_HIP(ws.buf[k].reserve(need[k]));
+  void* slot[S_NUM];
+  for (int k = 0; k < S_NUM; ++k) slot[k] = ws.buf[k].p;
+  return bind_batch(slot, bp, p, pts, res, s, bb, ra, true);
 }
 
 template <class P, class Pts>
@@ -316,6 +329,12 @@ template WorkspaceShape workspace_shape(const BatchPlan&, const clue_params*, co
                                        const clue_results*);
 template WorkspaceShape workspace_shape(const BatchPlan&, const clue_params_f64*,
                                        const clue_points_f64*, const clue_results_f64*);
+template clue_status bind_batch(void* const (&)[S_NUM], const BatchPlan&, const clue_params*,
+                                const clue_points*, clue_results*, hipStream_t,
+                                BatchBuffersT<float>*, ResultArraysT<float>*, bool);
+template clue_status bind_batch(void* const (&)[S_NUM], const BatchPlan&, const clue_params_f64*,
+                                const clue_points_f64*, clue_results_f64*, hipStream_t,
+                                BatchBuffersT<double>*, ResultArraysT<double>*, bool);
 template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params*,
                                    const clue_points*, clue_results*, hipStream_t,
                                    BatchBuffersT<float>*, ResultArraysT<float>*);

@@ -118,6 +118,22 @@ clue_status clue_make_clusters(clue_ctx* ctx, const clue_params* params,
 clue_status clue_make_clusters_async(clue_ctx* ctx, const clue_params* params,
                                      const clue_points* points, clue_results* results);
 
+/* Allocation-free variant of clue_make_clusters. NO COUNTERPART IN THE REFERENCE:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments,
+ * results and wait as clue_make_clusters, but the pipeline workspace is a static
+ * device array of 160 MiB per GPU (held until the process ends; its address is
+ * taken when the first context on the device is created) instead of the
+ * context's pooled buffers: the call makes no hipMalloc, hipFree, memset or
+ * host-to-device copy. One event of up to about a million points in 2-D or 3-D
+ * fits (fewer at higher ndim); larger events return CLUE_ERR_CAPACITY. Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * threads, other contexts, other streams) race and must be serialised by the
+ * caller. clue_get_seeds after it reads the array, so it is valid until the
+ * next such call on the device. */
+clue_status clue_make_clusters_noalloc(clue_ctx* ctx, const clue_params* params,
+                                       const clue_points* points, clue_results* results);
+
 /* Batched, device-resident: make_clusters(queue, d_points, batch_item_sizes, ...)
  * (core/detail/Clusterer.hpp:171-183, 286-350). Events are packed back to back in
  * `points`; sizes (HOST array) gives the size of each. Each event is tiled as if
