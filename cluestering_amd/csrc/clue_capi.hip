@@ -75,9 +75,13 @@ void* scratch_ptr(int device) {
 // The descriptors of one event of n points at p_off 0 (what the host plan holds),
 // written on the device: chunk c covers [c * kChunk, min(n, (c + 1) * kChunk)).
 __global__ void k_single_event_descriptors(EventDesc ev, EventDesc* __restrict__ events,
-                                           Chunk* __restrict__ chunks, int n_chunks) {
+                                           int32_t* __restrict__ order, Chunk* __restrict__ chunks,
+                                           int n_chunks) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c == 0) events[0] = ev;
+  if (c == 0) {
+    events[0] = ev;
+    if (order) order[0] = 0;
+  }
   if (c < n_chunks) {
     Chunk ch{};
     ch.event = 0;
@@ -228,6 +232,7 @@ clue_status clue_context_create(int32_t device, void* stream, clue_ctx** out) {
   // room for any event the scratch can hold, so that planning it allocates nothing.
   (void)init_scratch(device);
   c->plan.events.reserve(1);
+  c->plan.event_order.reserve(1);
   c->plan.chunks.reserve(kNoAllocScratchBytes / kChunk / 64 + 1);
   *out = c.release();
   return CLUE_OK;
@@ -338,6 +343,7 @@ clue_status clue_make_clusters_noalloc(clue_ctx* ctx, const clue_params* params,
   if (st) return st;
   hipLaunchKernelGGL(k_single_event_descriptors, dim3(static_cast<unsigned>((n_chunks + 255) / 256)),
                      dim3(256), 0, ctx->stream, bp.events[0], at<EventDesc>(slot, S_EVENTS),
+                     bp.event_order.empty() ? nullptr : at<int32_t>(slot, S_EORDER),
                      at<Chunk>(slot, S_CHUNKS), static_cast<int>(n_chunks));
   const RunParams rp = make_run_params(params, points, bp);
   for (bool& r : ctx->ev_recorded) r = false;

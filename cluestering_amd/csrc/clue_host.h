@@ -36,12 +36,13 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_NUM
 };
 
 // Host-side description of one batch (all derived from the event sizes).
 struct BatchPlan {
   std::vector<EventDesc> events;
+  std::vector<int32_t> event_order;  // fused per-event tile build: events largest first; empty: radix path
   std::vector<Chunk> chunks;
   int64_t n_total = 0;
   int64_t n_tiles_total = 0;
@@ -122,6 +123,16 @@ inline clue_status plan_batch(int D, int points_per_tile, const uint32_t* sizes,
   }
   bp->n_total = off;
   bp->n_tiles_total = tiles;
+  // The fused per-event tile build when every event's tiles fit its counters.
+  bool local = !bp->events.empty();
+  for (const EventDesc& ev : bp->events) local = local && ev.ntiles <= clue_dev::kEvtMaxTiles;
+  bp->event_order.clear();
+  if (local) {
+    for (int32_t k = 0; k < static_cast<int32_t>(bp->events.size()); ++k) bp->event_order.push_back(k);
+    if (bp->event_order.size() > 1)
+      std::stable_sort(bp->event_order.begin(), bp->event_order.end(),
+                       [&](int32_t a, int32_t b) { return bp->events[a].n > bp->events[b].n; });
+  }
   bp->tuning = fixed ? *fixed : table_tuning(D, off, static_cast<int64_t>(bp->events.size()));
   bp->max_items = (off + 63) / 64 + tiles;
   int bits = 0;
@@ -140,6 +151,7 @@ struct WorkspaceShape {
   int64_t chunks = 0;
   int64_t tiles = 0;        // sum of nper^D
   int64_t max_items = 0;
+  bool event_tiles = false;        // the fused per-event tile build (no radix-sort buffers)
   bool group_queries = false;
   bool separate_dm_order = false;  // dm != dc and the plain nearest-higher scan
   bool nh_blocks = false;          // nearest-higher over the Morton block layout
@@ -165,11 +177,12 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   const size_t D = static_cast<size_t>(w.D);
   bytes[S_EVENTS] = sizeof(EventDesc) * w.events;
   bytes[S_CHUNKS] = sizeof(Chunk) * w.chunks;
-  bytes[S_EXT] = static_cast<size_t>(w.key_bytes) * w.events * D * 2;
+  bytes[S_EXT] = w.event_tiles ? 0 : static_cast<size_t>(w.key_bytes) * w.events * D * 2;
   bytes[S_GEO] = T * w.events * 3 * D;
-  bytes[S_K0] = bytes[S_K1] = sizeof(uint32_t) * n;
-  bytes[S_V0] = bytes[S_V1] = sizeof(int32_t) * n;
-  bytes[S_HIST] = sizeof(int32_t) * hist;
+  bytes[S_K0] = bytes[S_K1] = w.event_tiles ? 0 : sizeof(uint32_t) * n;
+  bytes[S_V0] = bytes[S_V1] = w.event_tiles ? 0 : sizeof(int32_t) * n;
+  bytes[S_HIST] = w.event_tiles ? 0 : sizeof(int32_t) * hist;
+  bytes[S_EORDER] = w.event_tiles ? sizeof(int32_t) * w.events : 0;
   bytes[S_SCAN] = sizeof(int32_t) * scan_elems;
   bytes[S_TOFF] = sizeof(int32_t) * (NT + 1);
   bytes[S_ICOUNT] = sizeof(int32_t) * NT;

@@ -65,6 +65,7 @@ struct BatchBuffersT {
   int32_t max_items;      // upper bound of work items
   int32_t key_bits;
   const EventDesc* events;
+  const int32_t* event_order;  // events largest first for the fused tile build; NULL: radix path
   const Chunk* chunks;
   typename FloatBits<T>::Int* ext;  // [E][D][2] ordered-integer extremes
   T* geo;                           // [E][3][D] mn, mx, side

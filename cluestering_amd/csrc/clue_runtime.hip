@@ -181,6 +181,7 @@ WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, 
   w.chunks = static_cast<int64_t>(bp.chunks.size());
   w.tiles = bp.n_tiles_total;
   w.max_items = bp.max_items;
+  w.event_tiles = !bp.event_order.empty();
   w.group_queries = bp.tuning.group_queries != 0;
   w.nh_blocks = nh_blocks_enabled(p, pts, bp);
   w.separate_dm_order = p->dm != p->dc && !w.nh_blocks;
@@ -217,6 +218,9 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
                             hipMemcpyHostToDevice, s));
     CLUE_HIP(hipMemcpyAsync(slot[S_CHUNKS], bp.chunks.data(), sizeof(Chunk) * C,
                             hipMemcpyHostToDevice, s));
+    if (!bp.event_order.empty())
+      CLUE_HIP(hipMemcpyAsync(slot[S_EORDER], bp.event_order.data(), sizeof(int32_t) * E,
+                              hipMemcpyHostToDevice, s));
   }
   auto& b = *bb;
   b = BatchBuffersT<T>{};
@@ -227,6 +231,7 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   b.max_items = static_cast<int32_t>(bp.max_items);
   b.key_bits = bp.key_bits;
   b.events = at<EventDesc>(slot, S_EVENTS);
+  b.event_order = bp.event_order.empty() ? nullptr : at<int32_t>(slot, S_EORDER);
   b.chunks = at<Chunk>(slot, S_CHUNKS);
   b.ext = at<I>(slot, S_EXT);
   b.geo = at<T>(slot, S_GEO);

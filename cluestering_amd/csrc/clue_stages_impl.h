@@ -221,6 +221,165 @@ __global__ __launch_bounds__(kBlock) void k_gather(const GatherArgs<T> a, const 
     for (int k = key + 1; k <= n_tiles_total; ++k) toff[k] = n;
 }
 
+// ---------------------------------------------------------------------------
+// (a1-a3 fused) One workgroup per event when every event of the batch has at
+// most kEvtMaxTiles tiles: extremes and geometry, tile ids, a stable counting
+// sort by tile into the sorted SoA copy and the tile offsets, all in one
+// kernel. Same results as k_extremes + k_geometry + k_tile_keys + the radix
+// sort + k_gather: the extremes are the same ordered-integer min/max (NaN
+// skipped), the bins the same get_bin, and the order (tile, input index).
+// Each wave owns a contiguous sixteenth of the event, walked in
+// rounds of 64; lanes with the same tile are ranked by ballot, per-wave tile
+// counters live in LDS, and waves, rounds and lanes go in index order, so the
+// sort is stable. Events are taken largest first (order[]).
+// ---------------------------------------------------------------------------
+constexpr int kEvtWaves = 16;
+constexpr int kEvtBlock = kEvtWaves * kWave;
+static_assert(kEvtMaxTiles <= kEvtBlock, "one tile per thread in the offset scan");
+
+template <class T>
+__device__ __forceinline__ int event_tile_of(const GeoT<T>& g, const PointArraysT<T>& in, int D,
+                                             int i) {
+  int t = 0;
+  for (int d = 0; d < D; ++d) t = t * g.nper + get_bin(g, in.x[d][i], d);
+  return t;
+}
+
+template <class T>
+__global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const int32_t* __restrict__ order,
+    const PointArraysT<T> in, const SortedArraysT<T> out, T* __restrict__ geo,
+    int32_t* __restrict__ toff, int n_tiles_total, int n_total) {
+  using I = typename FloatBits<T>::Int;
+  constexpr I kHi = std::numeric_limits<I>::max(), kLo = std::numeric_limits<I>::min();
+  __shared__ int32_t cnt[kEvtWaves][kEvtMaxTiles];
+  __shared__ I red[kEvtWaves][2 * kMaxDim];
+  __shared__ T sgeo[3 * kMaxDim];
+  __shared__ int32_t wsum[kEvtWaves];
+  const int e = order[blockIdx.x];
+  const EventDesc ev = events[e];
+  const int D = rp.D;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int b0 = ev.p_off, n = ev.n;
+
+  // Extremes (ComputeTiles.hpp:17-36) and geometry (:33-34).
+  for (int d = 0; d < D; ++d) {
+    I lo = kHi, hi = kLo;
+    for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+      const T v = in.x[d][b0 + k];
+      if (v != v) continue;
+      const I o = float_to_ordered(v);
+      lo = o < lo ? o : lo;
+      hi = o > hi ? o : hi;
+    }
+    lo = wave_min_i(lo);
+    hi = wave_max_i(hi);
+    if (lane == 0) {
+      red[w][2 * d] = lo;
+      red[w][2 * d + 1] = hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < D) {
+    const int d = threadIdx.x;
+    I lo = kHi, hi = kLo;
+    for (int k = 0; k < kEvtWaves; ++k) {
+      lo = red[k][2 * d] < lo ? red[k][2 * d] : lo;
+      hi = red[k][2 * d + 1] > hi ? red[k][2 * d + 1] : hi;
+    }
+    const T mn = lo == kHi ? std::numeric_limits<T>::max() : ordered_to_float(lo);
+    const T mx = hi == kLo ? std::numeric_limits<T>::lowest() : ordered_to_float(hi);
+    const T side = (mx - mn) / static_cast<T>(ev.nper);
+    sgeo[d] = mn;
+    sgeo[D + d] = mx;
+    sgeo[2 * D + d] = side;
+    T* gq = geo + static_cast<int64_t>(e) * 3 * D;
+    gq[d] = mn;
+    gq[D + d] = mx;
+    gq[2 * D + d] = side;
+  }
+  for (int k = threadIdx.x; k < kEvtWaves * kEvtMaxTiles; k += kEvtBlock) (&cnt[0][0])[k] = 0;
+  __syncthreads();
+  GeoT<T> g;
+  g.D = D;
+  g.nper = ev.nper;
+  g.mn = sgeo;
+  g.mx = sgeo + D;
+  g.side = sgeo + 2 * D;
+  g.wrapped = rp.wrapped;
+
+  int bits = 0;
+  while ((1 << bits) < ev.ntiles) ++bits;
+  const int per_wave = (n + kEvtWaves - 1) / kEvtWaves;
+  const int wb = w * per_wave;
+  const int we = min(n, wb + per_wave);
+  const uint64_t lt = lanemask_lt();
+  // Per-wave tile counts.
+  for (int k0 = wb; k0 < we; k0 += kWave) {
+    const int k = k0 + lane;
+    const bool valid = k < we;
+    const uint32_t t = valid ? static_cast<uint32_t>(event_tile_of(g, in, D, b0 + k)) : 0u;
+    const uint64_t peers = match_peers(t, bits, valid);
+    if (valid && (peers & lt) == 0) cnt[w][t] += __popcll(peers);
+  }
+  __syncthreads();
+  // Tile offsets: exclusive scan over tiles (one per thread), then each wave's
+  // first position within each tile.
+  {
+    const int t = threadIdx.x;
+    int tot = 0;
+    if (t < ev.ntiles)
+      for (int k = 0; k < kEvtWaves; ++k) tot += cnt[k][t];
+    int incl = tot;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
+    }
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int k = 0; k < w; ++k) before += wsum[k];
+    const int base = before + incl - tot;
+    if (t < ev.ntiles) {
+      toff[ev.t_base + t] = b0 + base;
+      int run = base;
+      for (int k = 0; k < kEvtWaves; ++k) {
+        const int c = cnt[k][t];
+        cnt[k][t] = run;
+        run += c;
+      }
+    }
+    if (t == 0 && ev.t_base + ev.ntiles == n_tiles_total) toff[n_tiles_total] = n_total;
+  }
+  __syncthreads();
+  // Stable placement into the sorted copy.
+  for (int k0 = wb; k0 < we; k0 += kWave) {
+    const int k = k0 + lane;
+    const bool valid = k < we;
+    const int i = b0 + k;
+    const uint32_t t = valid ? static_cast<uint32_t>(event_tile_of(g, in, D, i)) : 0u;
+    const uint64_t peers = match_peers(t, bits, valid);
+    int pos = 0;
+    if (valid) pos = cnt[w][t] + __popcll(peers & lt);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lt) == 0) cnt[w][t] += __popcll(peers);
+    if (valid) {
+      const int s = b0 + pos;
+      for (int d = 0; d < D; ++d) out.x[d][s] = in.x[d][i];
+      const T wi = in.w[i];
+      out.w[s] = wi;
+      if (out.fw) out.fw[s] = rp.kp0 * wi;  // flat kernel: K * w for j != i
+      if (rp.has_sigma)
+        for (int d = 0; d < D; ++d) out.sigma[d][s] = in.sigma[d][i];
+      if (rp.has_u) out.u[s] = in.u[i];
+      out.tag[s] = rp.has_tags ? in.tags[i] : static_cast<uint32_t>(i);
+      out.perm[s] = i;
+    }
+  }
+}
+
 // Work items: ceil(count / 64) per tile.
 static __global__ void k_item_count(const int32_t* __restrict__ toff, int n_tiles,
                              int32_t* __restrict__ icount) {
@@ -1543,6 +1702,13 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
 
   // (a) tiles
   mark(kStageExtremes);
+  if (bb.event_order) {
+    // Every event fits one workgroup's tile counters: the fused per-event build.
+    mark(kStageTiles);
+    hipLaunchKernelGGL(k_event_tiles<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock), 0,
+                       s, rp, bb.events, bb.event_order, in, bb.sorted, bb.geo, bb.toff,
+                       bb.n_tiles_total, n);
+  } else {
   hipLaunchKernelGGL(k_init_extremes<I>,
                      dim3(static_cast<unsigned>((static_cast<int64_t>(bb.n_events) * D + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, s, bb.ext, static_cast<int64_t>(bb.n_events) * D);
@@ -1563,6 +1729,7 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   ga.cs = 0;
   hipLaunchKernelGGL(k_gather<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ga, rp, skeys,
                      sperm, n, bb.n_tiles_total, bb.toff);
+  }
   hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      s, bb.toff, bb.n_tiles_total, bb.icount);
   scan_exclusive(bb.icount, bb.icount, bb.n_tiles_total, bb.scan_tmp, bb.n_items, s);

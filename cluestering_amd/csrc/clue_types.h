@@ -10,6 +10,11 @@ constexpr int kMaxDim = 10;
 // one largest key per block).
 constexpr int kNhBlock = 16;
 
+// Largest tile count of an event that the fused per-event tile build takes (its
+// per-wave tile counters live in LDS); batches with a larger event take the
+// radix-sort path.
+constexpr int kEvtMaxTiles = 1024;
+
 // Workspace sizes of the scan and radix-sort primitives (clue_primitives.hip),
 // here so that host-only code sizes workspaces exactly as the device path does.
 constexpr int kScanTileElems = 4096;  // elements per block of the exclusive scan
