@@ -68,10 +68,10 @@ __device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool val
 }
 
 // ---------------------------------------------------------------------------
-// exp: Cody-Waite reduction x = k ln2 + r, degree-7 Taylor polynomial of e^r in
-// Horner form with fused multiply-adds, scaling by 2^k. Same method, constants
-// and operation order as the CPU statement of the algorithm, so both give the
-// same bits.
+// exp: Cody-Waite reduction x = k ln2 + r, the degree-6 minimax polynomial of
+// e^r on [-ln2/2, ln2/2] (constant and linear terms 1) in Horner form with fused
+// multiply-adds, scaling by 2^k. Same method, constants and operation order as
+// the CPU statement of the algorithm, so both give the same bits.
 // ---------------------------------------------------------------------------
 // The core for x in [LO, HI] = [-103.97, 88.72]: no range handling.
 __device__ __forceinline__ float expf_cw_core(float x) {
@@ -81,12 +81,11 @@ __device__ __forceinline__ float expf_cw_core(float x) {
   const float k = __builtin_rintf(x * log2e);
   float r = __builtin_fmaf(-k, ln2_hi, x);
   r = __builtin_fmaf(-k, ln2_lo, r);
-  float p = 0x1.a01a02p-13f;
-  p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
-  p = __builtin_fmaf(p, r, 0x1.111112p-7f);
-  p = __builtin_fmaf(p, r, 0x1.555556p-5f);
-  p = __builtin_fmaf(p, r, 0x1.555556p-3f);
-  p = __builtin_fmaf(p, r, 0x1.000000p-1f);
+  float p = 0x1.6a244cp-10f;
+  p = __builtin_fmaf(p, r, 0x1.1239d4p-7f);
+  p = __builtin_fmaf(p, r, 0x1.5558f2p-5f);
+  p = __builtin_fmaf(p, r, 0x1.555492p-3f);
+  p = __builtin_fmaf(p, r, 0x1.fffffcp-2f);
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
   return __builtin_ldexpf(p, static_cast<int>(k));

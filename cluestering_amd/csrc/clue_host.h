@@ -36,7 +36,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_ESEEDS, S_EOFF, S_NUM
 };
 
 // Host-side description of one batch (all derived from the event sizes).
@@ -123,9 +123,10 @@ inline clue_status plan_batch(int D, int points_per_tile, const uint32_t* sizes,
   }
   bp->n_total = off;
   bp->n_tiles_total = tiles;
-  // The fused per-event tile build when every event's tiles fit its counters.
+  // The fused per-event stages when every event's tiles and points fit their LDS.
   bool local = !bp->events.empty();
-  for (const EventDesc& ev : bp->events) local = local && ev.ntiles <= clue_dev::kEvtMaxTiles;
+  for (const EventDesc& ev : bp->events)
+    local = local && ev.ntiles <= clue_dev::kEvtMaxTiles && ev.n <= clue_dev::kEvtMaxPoints;
   bp->event_order.clear();
   if (local) {
     for (int32_t k = 0; k < static_cast<int32_t>(bp->events.size()); ++k) bp->event_order.push_back(k);
@@ -183,6 +184,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_V0] = bytes[S_V1] = w.event_tiles ? 0 : sizeof(int32_t) * n;
   bytes[S_HIST] = w.event_tiles ? 0 : sizeof(int32_t) * hist;
   bytes[S_EORDER] = w.event_tiles ? sizeof(int32_t) * w.events : 0;
+  bytes[S_ESEEDS] = bytes[S_EOFF] = w.event_tiles ? sizeof(int32_t) * w.events : 0;
   bytes[S_SCAN] = sizeof(int32_t) * scan_elems;
   bytes[S_TOFF] = sizeof(int32_t) * (NT + 1);
   bytes[S_ICOUNT] = sizeof(int32_t) * NT;
@@ -202,7 +204,8 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_SFW] = w.flat_kernel ? T * n : 0;
   bytes[S_SNH] = sizeof(int32_t) * n;
   bytes[S_SDELTA] = T * n;
-  bytes[S_FLAG] = bytes[S_RANK] = sizeof(int32_t) * n;
+  bytes[S_FLAG] = w.event_tiles ? 0 : sizeof(int32_t) * n;
+  bytes[S_RANK] = sizeof(int32_t) * n;
   bytes[S_NSEEDS] = sizeof(int32_t) * 4;
   bytes[S_SEEDS] = sizeof(int32_t) * n;
   bytes[S_SX] = T * stride * D;

@@ -88,6 +88,8 @@ struct BatchBuffersT {
   int32_t* snh;
   T* sdelta;
   int32_t* flag;       // [n_total]
+  int32_t* event_seeds;    // [n_events] seeds per event (fused per-event seed stage)
+  int32_t* event_offsets;  // [n_events] exclusive scan of event_seeds
   int32_t* rank;       // [n_total]
   int32_t* n_seeds;    // [1]
   int32_t* seeds;      // [n_total]

@@ -257,6 +257,8 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   b.snh = at<int32_t>(slot, S_SNH);
   b.sdelta = at<T>(slot, S_SDELTA);
   b.flag = at<int32_t>(slot, S_FLAG);
+  b.event_seeds = at<int32_t>(slot, S_ESEEDS);
+  b.event_offsets = at<int32_t>(slot, S_EOFF);
   b.rank = at<int32_t>(slot, S_RANK);
   b.n_seeds = at<int32_t>(slot, S_NSEEDS);
   b.seeds = at<int32_t>(slot, S_SEEDS);

@@ -237,15 +237,11 @@ constexpr int kEvtWaves = 16;
 constexpr int kEvtBlock = kEvtWaves * kWave;
 static_assert(kEvtMaxTiles <= kEvtBlock, "one tile per thread in the offset scan");
 
-template <class T>
-__device__ __forceinline__ int event_tile_of(const GeoT<T>& g, const PointArraysT<T>& in, int D,
-                                             int i) {
-  int t = 0;
-  for (int d = 0; d < D; ++d) t = t * g.nper + get_bin(g, in.x[d][i], d);
-  return t;
-}
+// Rounds of 64 points whose loads a wave issues together (memory-level
+// parallelism; the rounds are still ranked in order).
+constexpr int kEvtRounds = 4;
 
-template <class T>
+template <int D, class T>
 __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const int32_t* __restrict__ order,
     const PointArraysT<T> in, const SortedArraysT<T> out, T* __restrict__ geo,
@@ -253,25 +249,26 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
   using I = typename FloatBits<T>::Int;
   constexpr I kHi = std::numeric_limits<I>::max(), kLo = std::numeric_limits<I>::min();
   __shared__ int32_t cnt[kEvtWaves][kEvtMaxTiles];
-  __shared__ I red[kEvtWaves][2 * kMaxDim];
-  __shared__ T sgeo[3 * kMaxDim];
+  __shared__ I red[kEvtWaves][2 * D];
+  __shared__ T sgeo[3 * D];
   __shared__ int32_t wsum[kEvtWaves];
   const int e = order[blockIdx.x];
   const EventDesc ev = events[e];
-  const int D = rp.D;
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
   const int b0 = ev.p_off, n = ev.n;
 
   // Extremes (ComputeTiles.hpp:17-36) and geometry (:33-34).
+#pragma unroll
   for (int d = 0; d < D; ++d) {
     I lo = kHi, hi = kLo;
+#pragma unroll 4
     for (int k = threadIdx.x; k < n; k += kEvtBlock) {
       const T v = in.x[d][b0 + k];
-      if (v != v) continue;
       const I o = float_to_ordered(v);
-      lo = o < lo ? o : lo;
-      hi = o > hi ? o : hi;
+      const bool ok = v == v;  // NaN coordinates are skipped
+      lo = (ok && o < lo) ? o : lo;
+      hi = (ok && o > hi) ? o : hi;
     }
     lo = wave_min_i(lo);
     hi = wave_max_i(hi);
@@ -308,6 +305,13 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
   g.mx = sgeo + D;
   g.side = sgeo + 2 * D;
   g.wrapped = rp.wrapped;
+  // Tile id (TilesView.hpp:54-64) of the point with coordinates x.
+  auto tile_of = [&](const T (&x)[D]) {
+    int t = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) t = t * g.nper + get_bin(g, x[d], d);
+    return static_cast<uint32_t>(t);
+  };
 
   int bits = 0;
   while ((1 << bits) < ev.ntiles) ++bits;
@@ -316,12 +320,23 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
   const int we = min(n, wb + per_wave);
   const uint64_t lt = lanemask_lt();
   // Per-wave tile counts.
-  for (int k0 = wb; k0 < we; k0 += kWave) {
-    const int k = k0 + lane;
-    const bool valid = k < we;
-    const uint32_t t = valid ? static_cast<uint32_t>(event_tile_of(g, in, D, b0 + k)) : 0u;
-    const uint64_t peers = match_peers(t, bits, valid);
-    if (valid && (peers & lt) == 0) cnt[w][t] += __popcll(peers);
+  for (int k0 = wb; k0 < we; k0 += kWave * kEvtRounds) {
+    uint32_t t[kEvtRounds];
+    bool valid[kEvtRounds];
+#pragma unroll
+    for (int u = 0; u < kEvtRounds; ++u) {
+      const int k = k0 + u * kWave + lane;
+      valid[u] = k < we;
+      T x[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) x[d] = valid[u] ? in.x[d][b0 + k] : T(0);
+      t[u] = valid[u] ? tile_of(x) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kEvtRounds; ++u) {
+      const uint64_t peers = match_peers(t[u], bits, valid[u]);
+      if (valid[u] && (peers & lt) == 0) cnt[w][t[u]] += __popcll(peers);
+    }
   }
   __syncthreads();
   // Tile offsets: exclusive scan over tiles (one per thread), then each wave's
@@ -354,29 +369,68 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
     if (t == 0 && ev.t_base + ev.ntiles == n_tiles_total) toff[n_tiles_total] = n_total;
   }
   __syncthreads();
-  // Stable placement into the sorted copy.
-  for (int k0 = wb; k0 < we; k0 += kWave) {
-    const int k = k0 + lane;
-    const bool valid = k < we;
-    const int i = b0 + k;
-    const uint32_t t = valid ? static_cast<uint32_t>(event_tile_of(g, in, D, i)) : 0u;
-    const uint64_t peers = match_peers(t, bits, valid);
-    int pos = 0;
-    if (valid) pos = cnt[w][t] + __popcll(peers & lt);
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lt) == 0) cnt[w][t] += __popcll(peers);
-    if (valid) {
-      const int s = b0 + pos;
-      for (int d = 0; d < D; ++d) out.x[d][s] = in.x[d][i];
-      const T wi = in.w[i];
-      out.w[s] = wi;
-      if (out.fw) out.fw[s] = rp.kp0 * wi;  // flat kernel: K * w for j != i
-      if (rp.has_sigma)
-        for (int d = 0; d < D; ++d) out.sigma[d][s] = in.sigma[d][i];
-      if (rp.has_u) out.u[s] = in.u[i];
-      out.tag[s] = rp.has_tags ? in.tags[i] : static_cast<uint32_t>(i);
-      out.perm[s] = i;
+  // Stable placement into the sorted copy: rounds in order, lanes ranked by ballot.
+  for (int k0 = wb; k0 < we; k0 += kWave * kEvtRounds) {
+    T x[kEvtRounds][D];
+    T wt[kEvtRounds];
+    uint32_t t[kEvtRounds];
+    bool valid[kEvtRounds];
+#pragma unroll
+    for (int u = 0; u < kEvtRounds; ++u) {
+      const int k = k0 + u * kWave + lane;
+      valid[u] = k < we;
+#pragma unroll
+      for (int d = 0; d < D; ++d) x[u][d] = valid[u] ? in.x[d][b0 + k] : T(0);
+      wt[u] = valid[u] ? in.w[b0 + k] : T(0);
+      t[u] = valid[u] ? tile_of(x[u]) : 0u;
     }
+#pragma unroll
+    for (int u = 0; u < kEvtRounds; ++u) {
+      const uint64_t peers = match_peers(t[u], bits, valid[u]);
+      int pos = 0;
+      if (valid[u]) pos = cnt[w][t[u]] + __popcll(peers & lt);
+      __builtin_amdgcn_wave_barrier();
+      if (valid[u] && (peers & lt) == 0) cnt[w][t[u]] += __popcll(peers);
+      if (valid[u]) {
+        const int i = b0 + k0 + u * kWave + lane;
+        const int s = b0 + pos;
+#pragma unroll
+        for (int d = 0; d < D; ++d) out.x[d][s] = x[u][d];
+        out.w[s] = wt[u];
+        if (out.fw) out.fw[s] = rp.kp0 * wt[u];  // flat kernel: K * w for j != i
+        if (rp.has_sigma)
+#pragma unroll
+          for (int d = 0; d < D; ++d) out.sigma[d][s] = in.sigma[d][i];
+        if (rp.has_u) out.u[s] = in.u[i];
+        out.tag[s] = rp.has_tags ? in.tags[i] : static_cast<uint32_t>(i);
+        out.perm[s] = i;
+      }
+    }
+  }
+}
+
+template <int D, class T>
+static void launch_event_tiles_d(const RunParamsT<T>& rp, const PointArraysT<T>& in,
+                                 const BatchBuffersT<T>& bb, hipStream_t s) {
+  hipLaunchKernelGGL((k_event_tiles<D, T>), dim3(static_cast<unsigned>(bb.n_events)),
+                     dim3(kEvtBlock), 0, s, rp, bb.events, bb.event_order, in, bb.sorted, bb.geo,
+                     bb.toff, bb.n_tiles_total, bb.n_total);
+}
+
+template <class T>
+static void launch_event_tiles(const RunParamsT<T>& rp, const PointArraysT<T>& in,
+                               const BatchBuffersT<T>& bb, hipStream_t s) {
+  switch (rp.D) {
+    case 1: launch_event_tiles_d<1>(rp, in, bb, s); break;
+    case 2: launch_event_tiles_d<2>(rp, in, bb, s); break;
+    case 3: launch_event_tiles_d<3>(rp, in, bb, s); break;
+    case 4: launch_event_tiles_d<4>(rp, in, bb, s); break;
+    case 5: launch_event_tiles_d<5>(rp, in, bb, s); break;
+    case 6: launch_event_tiles_d<6>(rp, in, bb, s); break;
+    case 7: launch_event_tiles_d<7>(rp, in, bb, s); break;
+    case 8: launch_event_tiles_d<8>(rp, in, bb, s); break;
+    case 9: launch_event_tiles_d<9>(rp, in, bb, s); break;
+    default: launch_event_tiles_d<10>(rp, in, bb, s); break;
   }
 }
 
@@ -676,6 +730,71 @@ __device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* 
   }
 }
 
+__device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
+
+// First layout block of tile t whose candidates begin at sorted position b:
+// tile starts rounded up to whole blocks, one spare block per tile, so that the
+// tiles' block ranges never overlap.
+__device__ __forceinline__ int64_t nh_first_block(int b, int t) {
+  return static_cast<int64_t>((b + kNhBlock - 1) / kNhBlock) + t;
+}
+
+// The nearest-higher layout of tile t (one wave), written by the query-order
+// pass: the tile's candidates in Morton order (order[k], k < cnt) from block
+// nh_first_block on, padded to whole blocks (index -1), with each block's
+// bounding box. The keys, known after the density stage, are filled in by
+// k_nh_block_keys.
+template <int D, class T>
+__device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int32_t* order,
+                                                const T* __restrict__ sx, int64_t cs,
+                                                T* __restrict__ hx, int64_t hs,
+                                                int32_t* __restrict__ hidx, T* __restrict__ hbox) {
+  static_assert(kWave % kNhBlock == 0, "blocks must not straddle waves");
+  const int lane = lane_id();
+  const int64_t blk0 = nh_first_block(b, t);
+  const int span = (cnt + kNhBlock - 1) / kNhBlock * kNhBlock;
+  for (int k0 = 0; k0 < span; k0 += kWave) {
+    const int k = k0 + lane;
+    const bool real = k < cnt;
+    T x[D];
+    int s = -1;
+    if (real) s = order ? order[b + k] : b + k;
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = real ? sx[d * cs + s] : T(0);
+    const int64_t p = blk0 * kNhBlock + k;
+    if (k < span) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) hx[d * hs + p] = x[d];
+      hidx[p] = s;
+    }
+    T lo[D], hi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      lo[d] = real ? x[d] : std::numeric_limits<T>::infinity();
+      hi[d] = real ? x[d] : -std::numeric_limits<T>::infinity();
+    }
+#pragma unroll
+    for (int off = 1; off < kNhBlock; off <<= 1) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        lo[d] = fmin_t(lo[d], __shfl_xor(lo[d], off, 64));
+        hi[d] = fmax_t(hi[d], __shfl_xor(hi[d], off, 64));
+      }
+    }
+    if (k < span && (k % kNhBlock) == 0) {
+      const int64_t q = blk0 + k / kNhBlock;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        hbox[q * 2 * D + d] = lo[d];
+        hbox[q * 2 * D + D + d] = hi[d];
+      }
+    }
+  }
+}
+
 // Both query orders of one tile per wave. Pass 1 packs each point's keys into
 // qkey (Morton index in bits 0-7, class for dc in 8-13, class for dm in 14-19);
 // then three stable counting sorts: sorted order -> Morton order (qfine), and
@@ -687,7 +806,8 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
     const T* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
     const T* __restrict__ sx, int64_t cs, const T* __restrict__ ssig,
     uint32_t* __restrict__ qkey, int32_t* __restrict__ qfine, int32_t* __restrict__ qperm,
-    int32_t* __restrict__ qperm_nh) {
+    int32_t* __restrict__ qperm_nh, T* __restrict__ hx, int64_t hs, int32_t* __restrict__ hidx,
+    T* __restrict__ hbox) {
   constexpr int kD = D < kQueryDims ? D : kQueryDims;
   constexpr int kFineBits = (8 / kD) * kD;
   constexpr int kClassBits = 2 * kD;
@@ -703,6 +823,7 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       qperm[b + lane] = b + lane;
       qperm_nh[b + lane] = b + lane;
     }
+    if (hx) write_nh_layout<D>(t, b, e - b, nullptr, sx, cs, hx, hs, hidx, hbox);
     return;
   }
   // Event of tile t: last event with t_base <= t.
@@ -746,6 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
   if (qperm_nh != qperm)
     wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 14) & 63u; },
                                    qperm_nh);
+  if (hx) write_nh_layout<D>(t, b, e - b, qfine, sx, cs, hx, hs, hidx, hbox);
 }
 
 // ---------------------------------------------------------------------------
@@ -939,10 +1061,6 @@ __device__ __forceinline__ bool key_higher(uint64_t kj, uint32_t tj, uint64_t ki
                                            double) {
   return (kj > ki) | ((kj == ki) & (kj > 0x8000000000000000ull) & (tj > ti));
 }
-__device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fminf(a, b); }
-__device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
-__device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
-__device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ uint64_t query_key(float rho, uint32_t tag) {
   return rho_tag_key(rho, tag, true);
 }
@@ -1075,80 +1193,35 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
 // monotone), so the results are those of the plain scan.
 // ---------------------------------------------------------------------------
 
-// First layout block of tile t whose candidates begin at sorted position b:
-// tile starts rounded up to whole blocks, one spare block per tile, so that the
-// tiles' block ranges never overlap.
-__device__ __forceinline__ int64_t nh_first_block(int b, int t) {
-  return static_cast<int64_t>((b + kNhBlock - 1) / kNhBlock) + t;
-}
-
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
   return static_cast<uint64_t>(__shfl_xor(static_cast<long long>(v), m, 64));
 }
 
-// One wave per tile: the tile's candidates in Morton order (qfine) into the
-// layout, padding to whole blocks, then each block's box and largest key.
-template <int D, class T>
-__global__ __launch_bounds__(kBlock) void k_nh_layout(
-    const int32_t* __restrict__ toff, int n_tiles, const int32_t* __restrict__ qfine,
-    const T* __restrict__ sx, int64_t cs, const uint64_t* __restrict__ skey, T* __restrict__ hx,
-    int64_t hs, uint64_t* __restrict__ hkey, int32_t* __restrict__ hidx, T* __restrict__ hbox,
-    uint64_t* __restrict__ hbkey) {
-  static_assert(kWave % kNhBlock == 0, "blocks must not straddle waves");
-  const int t = blockIdx.x * kWaves + threadIdx.x / kWave;
-  if (t >= n_tiles) return;
-  const int lane = lane_id();
-  const int b = toff[t];
-  const int cnt = toff[t + 1] - b;
-  const int64_t blk0 = nh_first_block(b, t);
-  const int span = (cnt + kNhBlock - 1) / kNhBlock * kNhBlock;
-  for (int k0 = 0; k0 < span; k0 += kWave) {
-    const int k = k0 + lane;
-    const bool real = k < cnt;
-    T x[D];
-    uint64_t key = 0ull;
-    int s = -1;
-    if (real) {
-      s = qfine[b + k];
-      key = skey[s];
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = real ? sx[d * cs + s] : T(0);
-    const int64_t p = blk0 * kNhBlock + k;
-    if (k < span) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) hx[d * hs + p] = x[d];
-      hkey[p] = key;
-      hidx[p] = s;
-    }
-    // Box and largest key of each block: reductions over its kNhBlock lanes.
-    T lo[D], hi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      lo[d] = real ? x[d] : std::numeric_limits<T>::infinity();
-      hi[d] = real ? x[d] : -std::numeric_limits<T>::infinity();
-    }
-    uint64_t mk = key;
-#pragma unroll
-    for (int off = 1; off < kNhBlock; off <<= 1) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        lo[d] = fmin_t(lo[d], __shfl_xor(lo[d], off, 64));
-        hi[d] = fmax_t(hi[d], __shfl_xor(hi[d], off, 64));
-      }
-      const uint64_t o = shfl_xor_u64(mk, off);
-      mk = o > mk ? o : mk;
-    }
-    if (k < span && (k % kNhBlock) == 0) {
-      const int64_t q = blk0 + k / kNhBlock;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        hbox[q * 2 * D + d] = lo[d];
-        hbox[q * 2 * D + D + d] = hi[d];
-      }
-      hbkey[q] = mk;
-    }
+// Keys of the nearest-higher layout (the ordering key of each slot's point, 0 for
+// padding: never higher) and the largest key of every block, once the density
+// stage has written the keys. One thread per slot; a block's 16 slots are 16
+// consecutive lanes, reduced by shuffles. Blocks that belong to no tile hold
+// whatever the layout left in their index slots; they are never read.
+static __global__ __launch_bounds__(kBlock) void k_nh_block_keys(const int32_t* __restrict__ hidx,
+                                                                const uint64_t* __restrict__ skey,
+                                                                int64_t n_slots, int n_points,
+                                                                uint64_t* __restrict__ hkey,
+                                                                uint64_t* __restrict__ hbkey) {
+  static_assert(kBlock % kNhBlock == 0, "blocks must not straddle workgroups");
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  uint64_t k = 0ull;
+  if (p < n_slots) {
+    const int s = hidx[p];
+    k = (s >= 0 && s < n_points) ? skey[s] : 0ull;
+    hkey[p] = k;
   }
+  uint64_t m = k;
+#pragma unroll
+  for (int off = 1; off < kNhBlock; off <<= 1) {
+    const uint64_t o = shfl_xor_u64(m, off);
+    m = o > m ? o : m;
+  }
+  if (p < n_slots && (p % kNhBlock) == 0) hbkey[p / kNhBlock] = m;
 }
 
 // A lower bound, over every query point in [qlo, qhi] and every candidate inside
@@ -1481,6 +1554,97 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 }
 
 // ---------------------------------------------------------------------------
+// (d fused) With the per-event tile build, one workgroup per event does the seed
+// stage of k_unpermute, numbers the event's seeds in input order through a bit
+// per point in LDS (one popcount scan over the bits: the rank of a seed among
+// the event's seeds, stored in rank[i]), counts them (event_seeds[e]) and
+// shortens the follower links with the pointer-jumping passes of k_link_jump.
+// The seeds of earlier events are added by the label stage from an exclusive
+// scan of event_seeds.
+// ---------------------------------------------------------------------------
+constexpr int kLinkJumpPasses = 3;  // pointer-jumping passes before the label chase
+constexpr int kEvtBitWords = kEvtMaxPoints / 32;
+static_assert(kEvtBitWords % kEvtBlock == 0, "bit words per thread");
+
+template <class T>
+__global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const int32_t* __restrict__ order,
+    const int32_t* __restrict__ perm, const T* __restrict__ srho, int32_t* __restrict__ snh,
+    const T* __restrict__ sdelta, const T* __restrict__ su, T* __restrict__ rho,
+    int32_t* __restrict__ nh, T* __restrict__ delta, int32_t* __restrict__ is_seed,
+    int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds) {
+  constexpr int kWordsPerThread = kEvtBitWords / kEvtBlock;
+  __shared__ uint32_t bits[kEvtBitWords];
+  __shared__ int32_t wsum[kEvtWaves];
+  const int e = order[blockIdx.x];
+  const EventDesc ev = events[e];
+  const int b0 = ev.p_off, n = ev.n;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  for (int k = threadIdx.x; k < kEvtBitWords; k += kEvtBlock) bits[k] = 0u;
+  __syncthreads();
+  // Seed flags (ClusteringKernels.hpp:300-316), results in input order, links.
+#pragma unroll 4
+  for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+    const int s = b0 + k;
+    const int i = perm[s];
+    const T r = srho[s];
+    const int h = snh[s];
+    const T u = rp.has_u ? su[s] : T(1);
+    const bool seed = (h == -1) && (r >= rp.rhoc * u);
+    rho[i] = r;
+    nh[i] = h >= 0 ? perm[h] : -1;
+    if (delta) delta[i] = sdelta[s];
+    is_seed[i] = seed ? 1 : 0;
+    snh[s] = seed ? -2 : h;
+    if (seed) atomicOr(&bits[(i - b0) >> 5], 1u << ((i - b0) & 31));
+  }
+  __syncthreads();
+  // Ranks in input order: each thread owns kWordsPerThread consecutive words.
+  const int w0 = threadIdx.x * kWordsPerThread;
+  int mine = 0;
+#pragma unroll
+  for (int k = 0; k < kWordsPerThread; ++k) mine += __popc(bits[w0 + k]);
+  int incl = mine;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int k = 0; k < kEvtWaves; ++k) {
+    before += k < w ? wsum[k] : 0;
+    total += wsum[k];
+  }
+  int run = before + incl - mine;
+#pragma unroll
+  for (int k = 0; k < kWordsPerThread; ++k) {
+    uint32_t word = bits[w0 + k];
+    while (word) {
+      const int bit = __builtin_ctz(word);
+      word &= word - 1u;
+      rank[b0 + (w0 + k) * 32 + bit] = run++;
+    }
+  }
+  if (threadIdx.x == 0) event_seeds[e] = total;
+  // Pointer jumping on the event's links (as k_link_jump; any interleaving of the
+  // updates leaves every link on its own chain).
+  for (int pass = 0; pass < kLinkJumpPasses; ++pass) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+      const int s = b0 + k;
+      const int l = snh[s];
+      if (l < 0) continue;
+      const int l2 = snh[l];
+      if (l2 >= 0) snh[s] = l2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // (e) Labels: seeds take their rank in input order (the serial push_back order,
 // ClusteringKernels.hpp:310, 318-327); followers chase nearest_higher to a seed
 // or to a point without one (:329-343). The chase runs in tile-sorted space,
@@ -1500,20 +1664,26 @@ static __global__ __launch_bounds__(kBlock) void k_link_jump(int32_t* __restrict
   const int l2 = link[l];
   if (l2 >= 0) link[s] = l2;
 }
-constexpr int kLinkJumpPasses = 3;
 
+// rank[i] of a seed i is its global rank (seeds of the whole batch before it in
+// input order), or, with event_offsets, its rank within its event, to which the
+// seeds of the earlier events (event_offsets[e]) are added here.
 static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
                                                    const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ link,
                                                    const int32_t* __restrict__ rank,
+                                                   const int32_t* __restrict__ event_offsets,
                                                    int32_t* __restrict__ cluster,
                                                    int32_t* __restrict__ seeds,
                                                    int32_t* __restrict__ event_of_cluster,
                                                    int local_ids) {
   const Chunk ch = chunks[blockIdx.x];
-  // With per-event numbering, subtract the seeds of the earlier events.
-  const int base = local_ids ? rank[events[ch.event].p_off] : 0;
+  // Seeds of the earlier events: added to local ranks, subtracted from global
+  // ranks with per-event numbering.
+  const int first = event_offsets ? event_offsets[ch.event] : rank[events[ch.event].p_off];
+  const int add = event_offsets ? first : 0;
+  const int sub = local_ids ? first : 0;
   for (int s = ch.begin + threadIdx.x; s < ch.end; s += kBlock) {
     int cur = s;
     int nx = link[cur];
@@ -1522,8 +1692,8 @@ static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restric
       nx = link[cur];
     }
     const int i = perm[s];
-    const int c = (nx == -2) ? rank[perm[cur]] : -1;
-    cluster[i] = c >= 0 ? c - base : -1;
+    const int c = (nx == -2) ? rank[perm[cur]] + add : -1;
+    cluster[i] = c >= 0 ? c - sub : -1;
     if (cur == s && nx == -2) {
       seeds[c] = i;
       if (event_of_cluster) event_of_cluster[c] = events[ch.event].index;
@@ -1582,13 +1752,14 @@ static void launch_nh_dm(const NeighbourLaunch<T>& L) {
                      b.snh, b.sdelta);
 }
 
-template <int D, class T>
-static void launch_nh_layout_d(const NeighbourLaunch<T>& L) {
+// Largest key of each nearest-higher layout block, once the density stage has
+// written the keys.
+template <class T>
+static void launch_nh_block_keys(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
-  const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
-  hipLaunchKernelGGL((k_nh_layout<D, T>), dim3(grid), dim3(kBlock), 0, L.s, b.toff,
-                     b.n_tiles_total, b.qfine, b.sorted.x[0], L.cs, b.skey, b.hx, b.h_stride,
-                     b.hkey, b.hidx, b.hbox, b.hbkey);
+  const int64_t n_slots = b.h_stride;
+  hipLaunchKernelGGL(k_nh_block_keys, dim3(static_cast<unsigned>((n_slots + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, b.hidx, b.skey, n_slots, b.n_total, b.hkey, b.hbkey);
 }
 
 template <int D, int M, class T>
@@ -1605,7 +1776,7 @@ static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
 // blocks its lanes want, so grouping lanes by search-box class buys nothing here.
 template <int D, class T>
 static void launch_nh_blocks_d(const NeighbourLaunch<T>& L) {
-  launch_nh_layout_d<D>(L);
+  launch_nh_block_keys(L);
   switch (L.rp->metric) {
     case M_EUCLID: launch_nh_blocks_dm<D, M_EUCLID>(L); break;
     case M_WEUCLID: launch_nh_blocks_dm<D, M_WEUCLID>(L); break;
@@ -1651,7 +1822,8 @@ static void launch_query_order_d(const NeighbourLaunch<T>& L) {
   const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
   hipLaunchKernelGGL((k_query_order<D, T>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
                      b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
-                     b.sorted.sigma[0], b.qkey, b.qfine, b.qperm, b.qperm_nh);
+                     b.sorted.sigma[0], b.qkey, b.qfine, b.qperm, b.qperm_nh, b.hx, b.h_stride,
+                     b.hidx, b.hbox);
 }
 
 // The query orders of both neighbour stages (qperm for dc, qperm_nh for dm).
@@ -1705,9 +1877,7 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   if (bb.event_order) {
     // Every event fits one workgroup's tile counters: the fused per-event build.
     mark(kStageTiles);
-    hipLaunchKernelGGL(k_event_tiles<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock), 0,
-                       s, rp, bb.events, bb.event_order, in, bb.sorted, bb.geo, bb.toff,
-                       bb.n_tiles_total, n);
+    launch_event_tiles(rp, in, bb, s);
   } else {
   hipLaunchKernelGGL(k_init_extremes<I>,
                      dim3(static_cast<unsigned>((static_cast<int64_t>(bb.n_events) * D + kBlock - 1) / kBlock)),
@@ -1755,16 +1925,25 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
 
   // (d), (e)
   mark(kStageSeeds);
-  hipLaunchKernelGGL(k_unpermute<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
-                     bb.sorted.perm, bb.srho, bb.snh, bb.sdelta, bb.sorted.u, out.rho, out.nh,
-                     out.delta, out.is_seed, bb.flag);
-  scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
-  mark(kStageLabels);
-  for (int pass = 0; pass < kLinkJumpPasses; ++pass)
-    hipLaunchKernelGGL(k_link_jump, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, bb.snh, n);
+  if (bb.event_order) {
+    hipLaunchKernelGGL(k_event_seeds<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock),
+                       0, s, rp, bb.events, bb.event_order, bb.sorted.perm, bb.srho, bb.snh,
+                       bb.sdelta, bb.sorted.u, out.rho, out.nh, out.delta, out.is_seed, bb.rank,
+                       bb.event_seeds);
+    scan_exclusive(bb.event_seeds, bb.event_offsets, bb.n_events, bb.scan_tmp, bb.n_seeds, s);
+    mark(kStageLabels);
+  } else {
+    hipLaunchKernelGGL(k_unpermute<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
+                       bb.sorted.perm, bb.srho, bb.snh, bb.sdelta, bb.sorted.u, out.rho, out.nh,
+                       out.delta, out.is_seed, bb.flag);
+    scan_exclusive(bb.flag, bb.rank, n, bb.scan_tmp, bb.n_seeds, s);
+    mark(kStageLabels);
+    for (int pass = 0; pass < kLinkJumpPasses; ++pass)
+      hipLaunchKernelGGL(k_link_jump, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, bb.snh, n);
+  }
   hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, bb.events,
-                     bb.sorted.perm, bb.snh, bb.rank, out.cluster, bb.seeds,
-                     out.event_of_cluster, out.local_ids);
+                     bb.sorted.perm, bb.snh, bb.rank, bb.event_order ? bb.event_offsets : nullptr,
+                     out.cluster, bb.seeds, out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
 

@@ -14,6 +14,8 @@ constexpr int kNhBlock = 16;
 // per-wave tile counters live in LDS); batches with a larger event take the
 // radix-sort path.
 constexpr int kEvtMaxTiles = 1024;
+// Largest event the fused per-event stages take (one bit per point in LDS).
+constexpr int kEvtMaxPoints = 1 << 17;
 
 // Workspace sizes of the scan and radix-sort primitives (clue_primitives.hip),
 // here so that host-only code sizes workspaces exactly as the device path does.

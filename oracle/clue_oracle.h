@@ -25,8 +25,8 @@
  * bits are host dependent; this oracle fixes the strict IEEE evaluation.
  *
  * exp: the reference calls std::exp (glibc) on the host; its last-ulp bits are
- * library dependent. The oracle uses clue_expf (Cody-Waite reduction + degree-7
- * Taylor polynomial, fmaf Horner, see oracle_expf), pinned against libm's double
+ * library dependent. The oracle uses clue_expf (Cody-Waite reduction + degree-6
+ * minimax polynomial, fmaf Horner, see oracle_expf), pinned against libm's double
  * exp to <= 1 ulp in the tests. The same published method is what the device
  * kernels implement.
  */
