@@ -343,9 +343,11 @@ def main():
         rates = [float(r) for r in args.serve_rates.split(",") if r]
         served = []
         with Server(n_lanes=2, hbm_budget_bytes=64 << 30) as srv:
-            run_stream(srv, pool, rate=2000.0, n_events=200, seed=99)  # warm the lane pools
+            # Warm-up at the top rate: the lanes grow once to the largest batches the
+            # sweep forms, so no rate below pays for buffer growth.
+            run_stream(srv, pool, rate=max(rates), n_events=3000, seed=99)
             for rate in rates:
-                n_ev = int(min(max(rate * 1.5, 400), 6000))
+                n_ev = int(min(max(rate * 2.0, 2000), 8000))
                 r = run_stream(srv, pool, rate=rate, n_events=n_ev)
                 row = {k: (round(v, 3) if isinstance(v, float) else v)
                        for k, v in r.items() if k != "latency_ms"}

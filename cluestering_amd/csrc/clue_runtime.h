@@ -27,32 +27,39 @@ const char* last_error();
       return ::clue_rt::fail(CLUE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Device buffer that only grows.
+// Device buffer that only grows. With a stream and a memory pool set, growing
+// and releasing are stream-ordered (hipMallocFromPoolAsync / hipFreeAsync on that
+// stream): they do not wait for the whole device the way hipFree does, and the
+// pool keeps freed blocks for later growth instead of handing them back to the
+// driver. Such a buffer is used on that stream only, and its owner calls
+// release() before the stream and the pool go away.
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
+  hipStream_t stream = nullptr;
+  hipMemPool_t pool = nullptr;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && !pool) (void)hipFree(p);
   }
   hipError_t reserve(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
     if (p) {
-      hipError_t e = hipFree(p);
+      hipError_t e = pool ? hipFreeAsync(p, stream) : hipFree(p);
       if (e != hipSuccess) return e;
       p = nullptr;
       cap = 0;
     }
     const size_t want = grown_capacity(0, bytes);
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = pool ? hipMallocFromPoolAsync(&p, want, pool, stream) : hipMalloc(&p, want);
     if (e == hipSuccess) cap = want;
     return e;
   }
   // Frees the buffer (capacity back to 0).
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)(pool ? hipFreeAsync(p, stream) : hipFree(p));
     p = nullptr;
     cap = 0;
   }
@@ -64,6 +71,13 @@ struct DevBuf {
 
 struct Workspace {
   DevBuf buf[S_NUM];
+  // Stream-ordered growth of every buffer on stream s from pool mp (see DevBuf).
+  void set_stream(hipStream_t s, hipMemPool_t mp) {
+    for (DevBuf& b : buf) {
+      b.stream = s;
+      b.pool = mp;
+    }
+  }
 };
 
 // The C structs of each precision and the point type they carry.

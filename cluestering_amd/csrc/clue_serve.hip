@@ -56,6 +56,10 @@ __global__ void k_repack(const ReqDesc* __restrict__ reqs, const clue_dev::Chunk
 // buffer each lane pins on the host.
 constexpr int64_t kMaxLanePoints = int64_t(16) << 20;
 
+// Device staging bytes of a batch whose columns take col_bytes: each event may
+// bring up to 255 bytes of its ring block's padding.
+inline size_t staging_bytes_for(size_t col_bytes, size_t events) { return col_bytes + 256 * events; }
+
 // Device bytes of one lane for a batch of this shape: the pipeline workspace (the
 // request table prepare_batch reserves from) plus the lane's own staging,
 // columns, labels and descriptor buffers, each as its grow-only buffer would hold
@@ -78,7 +82,8 @@ int64_t lane_bytes(const clue_params& p, const clue_srv::BatchShape& shape, cons
   int64_t total = 0;
   for (int k = 0; k < S_NUM; ++k) total += grown_capacity(ws_caps ? ws_caps[k] : 0, need[k]);
   const size_t col_bytes = sizeof(float) * static_cast<size_t>(shape.points) * (p.ndim + 1);
-  const size_t own[4] = {col_bytes, col_bytes, sizeof(int32_t) * static_cast<size_t>(shape.points),
+  const size_t own[4] = {staging_bytes_for(col_bytes, static_cast<size_t>(shape.events)), col_bytes,
+                         sizeof(int32_t) * static_cast<size_t>(shape.points),
                          sizeof(ReqDesc) * static_cast<size_t>(shape.events)};
   for (int k = 0; k < 4; ++k) total += grown_capacity(own_caps ? own_caps[k] : 0, own[k]);
   return total;
@@ -106,18 +111,42 @@ class HipLanes : public clue_srv::ServeBackend {
   ~HipLanes() override {
     ScopedDevice dev(device_);
     for (auto& L : lanes_) {
-      if (L->stream) (void)hipStreamSynchronize(L->stream);
+      if (L->stream) {
+        release_buffers(*L);
+        (void)hipStreamSynchronize(L->stream);
+      }
       if (L->labels_host) (void)hipHostFree(L->labels_host);
       if (L->reqdesc_host) (void)hipHostFree(L->reqdesc_host);
       if (L->done) (void)hipEventDestroy(L->done);
       if (L->stream) (void)hipStreamDestroy(L->stream);
     }
+    if (pool_) (void)hipMemPoolDestroy(pool_);
+  }
+  // The lanes' device buffers come from one pool of this device that keeps what
+  // is freed (release threshold: no limit), so a lane that grows reuses blocks
+  // instead of mapping new memory, and no growth waits for the device.
+  clue_status init_pool() {
+    ScopedDevice dev(device_);
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.handleTypes = hipMemHandleTypeNone;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device_;
+    CLUE_HIP(hipMemPoolCreate(&pool_, &props));
+    uint64_t keep_all = UINT64_MAX;
+    CLUE_HIP(hipMemPoolSetAttribute(pool_, hipMemPoolAttrReleaseThreshold, &keep_all));
+    return CLUE_OK;
   }
   clue_status add_lane(int64_t lane_points) {
     auto L = std::make_unique<Lane>();
     L->owner = this;
     CLUE_HIP(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
     CLUE_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+    L->ws.set_stream(L->stream, pool_);
+    for (DevBuf* b : {&L->staging, &L->cols, &L->labels, &L->reqdesc}) {
+      b->stream = L->stream;
+      b->pool = pool_;
+    }
     CLUE_HIP(hipHostMalloc(reinterpret_cast<void**>(&L->labels_host),
                            sizeof(int32_t) * static_cast<size_t>(lane_points), hipHostMallocDefault));
     lanes_.push_back(std::move(L));
@@ -163,17 +192,25 @@ class HipLanes : public clue_srv::ServeBackend {
   int64_t bytes_fresh(const clue_params& p, const clue_srv::BatchShape& shape) const override {
     return lane_bytes(p, shape, nullptr, nullptr);
   }
+  // Gives the lane's device memory back to the driver: the frees are ordered on
+  // the lane's stream, which is idle here, and the pool then drops every block no
+  // lane holds.
   void trim(int lane) override {
     Lane& L = *lanes_[lane];
     ScopedDevice dev(device_);
+    release_buffers(L);
+    (void)hipStreamSynchronize(L.stream);
+    (void)hipMemPoolTrimTo(pool_, 0);
+  }
+
+ private:
+  static void release_buffers(Lane& L) {
     for (DevBuf& b : L.ws.buf) b.release();
     L.staging.release();
     L.cols.release();
     L.labels.release();
     L.reqdesc.release();
   }
-
- private:
   static void on_done(void* lane) { static_cast<Lane*>(lane)->owner->core_->notify(); }
 
   clue_status launch_impl(Lane& L, const std::vector<clue_srv::RequestP>& batch, const char* ring) {
@@ -191,7 +228,7 @@ class HipLanes : public clue_srv::ServeBackend {
     clue_status st = make_batch_plan(&p, sizes.data(), static_cast<int32_t>(sizes.size()), &L.plan);
     if (st) return st;
     const size_t col_bytes = sizeof(float) * static_cast<size_t>(n_total) * (D + 1);
-    CLUE_HIP(L.staging.reserve(col_bytes));
+    CLUE_HIP(L.staging.reserve(staging_bytes_for(col_bytes, batch.size())));
     CLUE_HIP(L.cols.reserve(col_bytes));
     CLUE_HIP(L.labels.reserve(sizeof(int32_t) * n_total));
     CLUE_HIP(L.reqdesc.reserve(sizeof(ReqDesc) * batch.size()));
@@ -204,16 +241,33 @@ class HipLanes : public clue_srv::ServeBackend {
     }
     // Every submitted event has n >= 1, so the plan describes the batch's events
     // in order.
-    float* staging = L.staging.as<float>();
-    int64_t src = 0;
+    // Requests that sit next to each other in the ring (admitted one after the
+    // other) go up in one copy: within such a run staging holds the ring's bytes,
+    // 256-byte block padding included, and runs follow each other.
+    char* staging_bytes = L.staging.as<char>();
+    size_t run_ring = 0, run_dst = 0, run_bytes = 0;
     for (size_t k = 0; k < batch.size(); ++k) {
       const clue_srv::Request& r = *batch[k];
       const size_t bytes = sizeof(float) * static_cast<size_t>(r.n) * (D + 1);
-      CLUE_HIP(hipMemcpyAsync(staging + src, ring + r.ring_off, bytes, hipMemcpyHostToDevice,
-                              L.stream));
-      L.reqdesc_host[k] = ReqDesc{src, r.n, L.offsets[k]};
-      src += static_cast<int64_t>(r.n) * (D + 1);
+      if (run_bytes > 0 && r.ring_off >= run_ring + run_bytes &&
+          r.ring_off - (run_ring + run_bytes) < 256) {
+        run_bytes = r.ring_off + bytes - run_ring;
+      } else {
+        if (run_bytes > 0) {
+          CLUE_HIP(hipMemcpyAsync(staging_bytes + run_dst, ring + run_ring, run_bytes,
+                                  hipMemcpyHostToDevice, L.stream));
+          run_dst += (run_bytes + 255) & ~size_t(255);
+        }
+        run_ring = r.ring_off;
+        run_bytes = bytes;
+      }
+      const size_t at = run_dst + (r.ring_off - run_ring);  // bytes into staging
+      L.reqdesc_host[k] = ReqDesc{static_cast<int64_t>(at / sizeof(float)), r.n, L.offsets[k]};
     }
+    if (run_bytes > 0)
+      CLUE_HIP(hipMemcpyAsync(staging_bytes + run_dst, ring + run_ring, run_bytes,
+                              hipMemcpyHostToDevice, L.stream));
+    float* staging = L.staging.as<float>();
     CLUE_HIP(hipMemcpyAsync(L.reqdesc.p, L.reqdesc_host, sizeof(ReqDesc) * batch.size(),
                             hipMemcpyHostToDevice, L.stream));
     clue_points pts{};
@@ -241,6 +295,7 @@ class HipLanes : public clue_srv::ServeBackend {
   }
 
   int device_;
+  hipMemPool_t pool_ = nullptr;
   std::vector<std::unique_ptr<Lane>> lanes_;
   clue_srv::ServeCore* core_ = nullptr;
 };
@@ -283,6 +338,10 @@ clue_status clue_server_create(const clue_server_config* cfg, clue_server** out)
   lim.max_batch_events = s->cfg.max_batch_events;
   lim.staging_bytes = static_cast<size_t>(s->cfg.staging_bytes);
   ScopedDevice dev(cfg->device);
+  {
+    const clue_status st = s->lanes.init_pool();
+    if (st) return st;
+  }
   for (int k = 0; k < s->cfg.n_lanes; ++k) {
     const clue_status st = s->lanes.add_lane(lim.lane_points);
     if (st) return st;

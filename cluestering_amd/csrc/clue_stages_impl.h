@@ -980,6 +980,10 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
       // the reference's addition of a zero term.
       const T next = rho + ((own && j == s) ? wself : wj);
       rho = (v <= thr) ? next : rho;
+    } else if constexpr (K == K_EXP_UNIT || K == K_GAUSS2_UNIT) {
+      // Amplitude 1: the diagonal's distance is 0 and exp(-0) is exactly 1, the
+      // reference's K(i, i), so the diagonal needs no case of its own.
+      if (v <= thr) rho += kernel_value<M, K>(v, rp) * wj;
     } else {
       if (v <= thr) {
         const T kv = (own && j == s) ? T(1) : kernel_value<M, K>(v, rp);
