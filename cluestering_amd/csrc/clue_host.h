@@ -36,7 +36,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_ESEEDS, S_EOFF, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_ESEEDS, S_EOFF, S_IPERM, S_NUM
 };
 
 // Host-side description of one batch (all derived from the event sizes).
@@ -185,6 +185,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_HIST] = w.event_tiles ? 0 : sizeof(int32_t) * hist;
   bytes[S_EORDER] = w.event_tiles ? sizeof(int32_t) * w.events : 0;
   bytes[S_ESEEDS] = bytes[S_EOFF] = w.event_tiles ? sizeof(int32_t) * w.events : 0;
+  bytes[S_IPERM] = w.event_tiles ? sizeof(int32_t) * n : 0;
   bytes[S_SCAN] = sizeof(int32_t) * scan_elems;
   bytes[S_TOFF] = sizeof(int32_t) * (NT + 1);
   bytes[S_ICOUNT] = sizeof(int32_t) * NT;

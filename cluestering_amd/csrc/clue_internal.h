@@ -90,6 +90,7 @@ struct BatchBuffersT {
   int32_t* flag;       // [n_total]
   int32_t* event_seeds;    // [n_events] seeds per event (fused per-event seed stage)
   int32_t* event_offsets;  // [n_events] exclusive scan of event_seeds
+  int32_t* iperm;          // [n_total] sorted position of every input point (fused per-event path)
   int32_t* rank;       // [n_total]
   int32_t* n_seeds;    // [1]
   int32_t* seeds;      // [n_total]

@@ -259,6 +259,7 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   b.flag = at<int32_t>(slot, S_FLAG);
   b.event_seeds = at<int32_t>(slot, S_ESEEDS);
   b.event_offsets = at<int32_t>(slot, S_EOFF);
+  b.iperm = bp.event_order.empty() ? nullptr : at<int32_t>(slot, S_IPERM);
   b.rank = at<int32_t>(slot, S_RANK);
   b.n_seeds = at<int32_t>(slot, S_NSEEDS);
   b.seeds = at<int32_t>(slot, S_SEEDS);

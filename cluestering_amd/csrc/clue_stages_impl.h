@@ -245,7 +245,7 @@ template <int D, class T>
 __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const int32_t* __restrict__ order,
     const PointArraysT<T> in, const SortedArraysT<T> out, T* __restrict__ geo,
-    int32_t* __restrict__ toff, int n_tiles_total, int n_total) {
+    int32_t* __restrict__ toff, int32_t* __restrict__ iperm, int n_tiles_total, int n_total) {
   using I = typename FloatBits<T>::Int;
   constexpr I kHi = std::numeric_limits<I>::max(), kLo = std::numeric_limits<I>::min();
   __shared__ int32_t cnt[kEvtWaves][kEvtMaxTiles];
@@ -369,20 +369,21 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
     if (t == 0 && ev.t_base + ev.ntiles == n_tiles_total) toff[n_tiles_total] = n_total;
   }
   __syncthreads();
-  // Stable placement into the sorted copy: rounds in order, lanes ranked by ballot.
+  // Stable placement: rounds in order, lanes ranked by ballot. Only the
+  // permutation is scattered (perm[s] = i, and iperm[i] = s in input order);
+  // the sorted columns are then gathered in sorted order, so their writes are
+  // whole lines instead of one scattered word per column.
   for (int k0 = wb; k0 < we; k0 += kWave * kEvtRounds) {
-    T x[kEvtRounds][D];
-    T wt[kEvtRounds];
     uint32_t t[kEvtRounds];
     bool valid[kEvtRounds];
 #pragma unroll
     for (int u = 0; u < kEvtRounds; ++u) {
       const int k = k0 + u * kWave + lane;
       valid[u] = k < we;
+      T x[D];
 #pragma unroll
-      for (int d = 0; d < D; ++d) x[u][d] = valid[u] ? in.x[d][b0 + k] : T(0);
-      wt[u] = valid[u] ? in.w[b0 + k] : T(0);
-      t[u] = valid[u] ? tile_of(x[u]) : 0u;
+      for (int d = 0; d < D; ++d) x[d] = valid[u] ? in.x[d][b0 + k] : T(0);
+      t[u] = valid[u] ? tile_of(x) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kEvtRounds; ++u) {
@@ -393,19 +394,27 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_tiles(
       if (valid[u] && (peers & lt) == 0) cnt[w][t[u]] += __popcll(peers);
       if (valid[u]) {
         const int i = b0 + k0 + u * kWave + lane;
-        const int s = b0 + pos;
-#pragma unroll
-        for (int d = 0; d < D; ++d) out.x[d][s] = x[u][d];
-        out.w[s] = wt[u];
-        if (out.fw) out.fw[s] = rp.kp0 * wt[u];  // flat kernel: K * w for j != i
-        if (rp.has_sigma)
-#pragma unroll
-          for (int d = 0; d < D; ++d) out.sigma[d][s] = in.sigma[d][i];
-        if (rp.has_u) out.u[s] = in.u[i];
-        out.tag[s] = rp.has_tags ? in.tags[i] : static_cast<uint32_t>(i);
-        out.perm[s] = i;
+        out.perm[b0 + pos] = i;
+        iperm[i] = b0 + pos;
       }
     }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // perm of the whole event
+  __syncthreads();
+#pragma unroll 4
+  for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+    const int s = b0 + k;
+    const int i = out.perm[s];
+#pragma unroll
+    for (int d = 0; d < D; ++d) out.x[d][s] = in.x[d][i];
+    const T wt = in.w[i];
+    out.w[s] = wt;
+    if (out.fw) out.fw[s] = rp.kp0 * wt;  // flat kernel: K * w for j != i
+    if (rp.has_sigma)
+#pragma unroll
+      for (int d = 0; d < D; ++d) out.sigma[d][s] = in.sigma[d][i];
+    if (rp.has_u) out.u[s] = in.u[i];
+    out.tag[s] = rp.has_tags ? in.tags[i] : static_cast<uint32_t>(i);
   }
 }
 
@@ -414,7 +423,7 @@ static void launch_event_tiles_d(const RunParamsT<T>& rp, const PointArraysT<T>&
                                  const BatchBuffersT<T>& bb, hipStream_t s) {
   hipLaunchKernelGGL((k_event_tiles<D, T>), dim3(static_cast<unsigned>(bb.n_events)),
                      dim3(kEvtBlock), 0, s, rp, bb.events, bb.event_order, in, bb.sorted, bb.geo,
-                     bb.toff, bb.n_tiles_total, bb.n_total);
+                     bb.toff, bb.iperm, bb.n_tiles_total, bb.n_total);
 }
 
 template <class T>
@@ -1463,7 +1472,11 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
   // arrive in one vector load (one block per lane); a wave-wide test against the
   // box, smallest key and largest bound of the lanes that hold the tile drops
   // most blocks at once, and each block left is tested per lane before its scan.
-  auto scan_tile = [&](int t, bool in) {
+  // The tile's blocks are visited from block sb on, wrapping around: in the own
+  // tile sb is the block that holds the wave's first point, whose neighbours
+  // settle most bounds before the rest of the tile is tested. The result does
+  // not depend on the order (the acceptance test is a strict total order).
+  auto scan_tile = [&](int t, bool in, int sb) {
     const int b = toff[t];
     const int nblk = (toff[t + 1] - b + kNhBlock - 1) / kNhBlock;
     const int64_t blk0 = nh_first_block(b, t);
@@ -1478,7 +1491,9 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
       const T bmax = wave_max_f(in ? bound : -std::numeric_limits<T>::infinity());
       const int l = lane_id();
       const bool valid = c0 + l < nblk;
-      const int64_t qb = blk0 + c0 + (valid ? l : 0);
+      int rl = c0 + (valid ? l : 0) + sb;
+      rl -= rl >= nblk ? nblk : 0;
+      const int64_t qb = blk0 + rl;
       T bl[D], bh[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -1499,7 +1514,9 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
         }
         const bool need = in & block_may_be_higher(read_lane(bk, i), key_i, T(0)) &
                           (metric_lb<D, M>(xi, xi, sl, sh, rp) <= bound);
-        if (ballot(need) != 0ull) scan_block((blk0 + c0 + i) * kNhBlock, in);
+        int ri = c0 + i + sb;
+        ri -= ri >= nblk ? nblk : 0;
+        if (ballot(need) != 0ull) scan_block((blk0 + ri) * kNhBlock, in);
       }
     }
   };
@@ -1519,10 +1536,10 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
       in_own = in_own && (((lo[d] <= b) && (b <= hi[d])) ||
                           (rp.wrapped[d] && (lo[d] <= b + ev.nper) && (b + ev.nper <= hi[d])));
     }
-    if (ballot(in_own) != 0ull) scan_tile(own, in_own);
+    if (ballot(in_own) != 0ull) scan_tile(own, in_own, uniform(it.begin - toff[own]) / kNhBlock);
   }
   walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
-    if (t != own) scan_tile(t, in);
+    if (t != own) scan_tile(t, in, 0);
   });
   if (active) {
     snh[s] = nh_slot >= 0 ? hidx[nh_slot] : -1;
@@ -1564,7 +1581,9 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // the event's seeds, stored in rank[i]), counts them (event_seeds[e]) and
 // shortens the follower links with the pointer-jumping passes of k_link_jump.
 // The seeds of earlier events are added by the label stage from an exclusive
-// scan of event_seeds.
+// scan of event_seeds. The seed flags and links are settled in sorted order;
+// the results are then written in input order, each point's sorted values
+// gathered through iperm, so that every store is coalesced.
 // ---------------------------------------------------------------------------
 constexpr int kLinkJumpPasses = 3;  // pointer-jumping passes before the label chase
 constexpr int kEvtBitWords = kEvtMaxPoints / 32;
@@ -1573,10 +1592,10 @@ static_assert(kEvtBitWords % kEvtBlock == 0, "bit words per thread");
 template <class T>
 __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const int32_t* __restrict__ order,
-    const int32_t* __restrict__ perm, const T* __restrict__ srho, int32_t* __restrict__ snh,
-    const T* __restrict__ sdelta, const T* __restrict__ su, T* __restrict__ rho,
-    int32_t* __restrict__ nh, T* __restrict__ delta, int32_t* __restrict__ is_seed,
-    int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds) {
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+    const T* __restrict__ srho, int32_t* __restrict__ snh, const T* __restrict__ sdelta,
+    const T* __restrict__ su, T* __restrict__ rho, int32_t* __restrict__ nh, T* __restrict__ delta,
+    int32_t* __restrict__ is_seed, int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds) {
   constexpr int kWordsPerThread = kEvtBitWords / kEvtBlock;
   __shared__ uint32_t bits[kEvtBitWords];
   __shared__ int32_t wsum[kEvtWaves];
@@ -1587,23 +1606,33 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
   const int lane = lane_id();
   for (int k = threadIdx.x; k < kEvtBitWords; k += kEvtBlock) bits[k] = 0u;
   __syncthreads();
-  // Seed flags (ClusteringKernels.hpp:300-316), results in input order, links.
+  // Seed flags (ClusteringKernels.hpp:300-316) in sorted order: a seed's link
+  // becomes -2, and its bit (input order) is set.
 #pragma unroll 4
   for (int k = threadIdx.x; k < n; k += kEvtBlock) {
     const int s = b0 + k;
-    const int i = perm[s];
-    const T r = srho[s];
     const int h = snh[s];
     const T u = rp.has_u ? su[s] : T(1);
-    const bool seed = (h == -1) && (r >= rp.rhoc * u);
-    rho[i] = r;
+    const bool seed = (h == -1) && (srho[s] >= rp.rhoc * u);
+    if (seed) {
+      snh[s] = -2;
+      const int i = perm[s] - b0;
+      atomicOr(&bits[i >> 5], 1u << (i & 31));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the -2 links
+  __syncthreads();
+  // Results in input order.
+#pragma unroll 4
+  for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+    const int i = b0 + k;
+    const int s = iperm[i];
+    const int h = snh[s];
+    rho[i] = srho[s];
     nh[i] = h >= 0 ? perm[h] : -1;
     if (delta) delta[i] = sdelta[s];
-    is_seed[i] = seed ? 1 : 0;
-    snh[s] = seed ? -2 : h;
-    if (seed) atomicOr(&bits[(i - b0) >> 5], 1u << ((i - b0) & 31));
+    is_seed[i] = h == -2 ? 1 : 0;
   }
-  __syncthreads();
   // Ranks in input order: each thread owns kWordsPerThread consecutive words.
   const int w0 = threadIdx.x * kWordsPerThread;
   int mine = 0;
@@ -1671,7 +1700,10 @@ static __global__ __launch_bounds__(kBlock) void k_link_jump(int32_t* __restrict
 
 // rank[i] of a seed i is its global rank (seeds of the whole batch before it in
 // input order), or, with event_offsets, its rank within its event, to which the
-// seeds of the earlier events (event_offsets[e]) are added here.
+// seeds of the earlier events (event_offsets[e]) are added here. Chains are
+// chased from sorted positions, so their first links are read in order and the
+// label stores scatter within each event (chasing from input order instead
+// stores in order but reads twice the bytes, for the same time on MI355X).
 static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
                                                    const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
@@ -1931,7 +1963,7 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   mark(kStageSeeds);
   if (bb.event_order) {
     hipLaunchKernelGGL(k_event_seeds<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock),
-                       0, s, rp, bb.events, bb.event_order, bb.sorted.perm, bb.srho, bb.snh,
+                       0, s, rp, bb.events, bb.event_order, bb.sorted.perm, bb.iperm, bb.srho, bb.snh,
                        bb.sdelta, bb.sorted.u, out.rho, out.nh, out.delta, out.is_seed, bb.rank,
                        bb.event_seeds);
     scan_exclusive(bb.event_seeds, bb.event_offsets, bb.n_events, bb.scan_tmp, bb.n_seeds, s);
@@ -1946,7 +1978,8 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
       hipLaunchKernelGGL(k_link_jump, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, bb.snh, n);
   }
   hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, bb.events,
-                     bb.sorted.perm, bb.snh, bb.rank, bb.event_order ? bb.event_offsets : nullptr,
+                     bb.sorted.perm, bb.snh, bb.rank,
+                     bb.event_order ? bb.event_offsets : nullptr,
                      out.cluster, bb.seeds, out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
