@@ -55,6 +55,32 @@ __device__ __forceinline__ long long wave_max_i(long long v) {
 }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Wave-wide reduction of a 32-bit value through DPP lane moves instead of LDS
+// permutes (whose results would share a wait counter with the scalar loads):
+// within each row of 16 lanes a pair swap, a quad swap, the half-row mirror and
+// the row mirror give every lane its row's result; the broadcasts of lane 15 to
+// rows 1 and 3 and of lane 31 to rows 2 and 3 then fold rows 0-2 into row 3, and
+// lane 63's value is returned in a scalar register. Lanes a move does not write
+// take the operation's identity id, so each move can fuse into its operation.
+// op must be commutative, associative and idempotent (min, max), and every lane
+// of the wave active.
+template <class F>
+__device__ __forceinline__ int wave_reduce_dpp(int v, int id, F op) {
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x142, 0xA, 0xF, false));  // row_bcast:15
+  v = op(v, __builtin_amdgcn_update_dpp(id, v, 0x143, 0xC, 0xF, false));  // row_bcast:31
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_min_uniform(int v) {
+  return wave_reduce_dpp(v, INT32_MAX, [](int a, int b) { return a < b ? a : b; });
+}
+__device__ __forceinline__ int wave_max_uniform(int v) {
+  return wave_reduce_dpp(v, INT32_MIN, [](int a, int b) { return a > b ? a : b; });
+}
+
 // Peers: the set of active lanes whose key equals this lane's key, built from one
 // ballot per key bit. Lanes with valid == false are excluded.
 __device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool valid) {
@@ -75,17 +101,14 @@ __device__ __forceinline__ uint64_t match_peers(uint32_t key, int bits, bool val
 // ---------------------------------------------------------------------------
 // The core for x in [LO, HI] = [-103.97, 88.72]: no range handling.
 __device__ __forceinline__ float expf_cw_core(float x) {
-  const float log2e = 0x1.715476p+0f;
-  const float ln2_hi = 0x1.62e400p-1f;
-  const float ln2_lo = 0x1.7f7d1cp-20f;
-  const float k = __builtin_rintf(x * log2e);
-  float r = __builtin_fmaf(-k, ln2_hi, x);
-  r = __builtin_fmaf(-k, ln2_lo, r);
-  float p = 0x1.6a244cp-10f;
-  p = __builtin_fmaf(p, r, 0x1.1239d4p-7f);
-  p = __builtin_fmaf(p, r, 0x1.5558f2p-5f);
-  p = __builtin_fmaf(p, r, 0x1.555492p-3f);
-  p = __builtin_fmaf(p, r, 0x1.fffffcp-2f);
+  const float k = __builtin_rintf(x * kExpLog2e);
+  float r = __builtin_fmaf(-k, kExpLn2Hi, x);
+  r = __builtin_fmaf(-k, kExpLn2Lo, r);
+  float p = kExpPoly[0];
+  p = __builtin_fmaf(p, r, kExpPoly[1]);
+  p = __builtin_fmaf(p, r, kExpPoly[2]);
+  p = __builtin_fmaf(p, r, kExpPoly[3]);
+  p = __builtin_fmaf(p, r, kExpPoly[4]);
   p = __builtin_fmaf(p, r, 1.0f);
   p = __builtin_fmaf(p, r, 1.0f);
   return __builtin_ldexpf(p, static_cast<int>(k));
@@ -94,12 +117,36 @@ __device__ __forceinline__ float expf_cw_core(float x) {
 // Branch-free for every x: the core runs on x clamped to [LO, HI] and the
 // out-of-range results are selected afterwards.
 __device__ __forceinline__ float expf_cw(float x) {
-  const float lo = -0x1.9fe368p+6f, hi = 0x1.62e42ep+6f;
   const float xin = x;
-  float e = expf_cw_core(__builtin_fminf(__builtin_fmaxf(x, lo), hi));
-  e = (xin > hi) ? __builtin_inff() : e;
-  e = (xin < lo) ? 0.0f : e;
+  float e = expf_cw_core(__builtin_fminf(__builtin_fmaxf(x, kExpCoreLo), kExpCoreHi));
+  e = (xin > kExpCoreHi) ? __builtin_inff() : e;
+  e = (xin < kExpCoreLo) ? 0.0f : e;
   return (xin != xin) ? xin : e;
+}
+
+// expf_cw_core(y * 2^-s) and expf_cw(y * 2^-s) with the scale folded into the
+// constants (ExpScaled): the same bits without the multiplication. A NaN y gives
+// y itself, the NaN that y * 2^-s would carry. c0 is e.c[0], which a caller in a
+// loop may hold in a vector register (the first step's other two operands are
+// scalar).
+__device__ __forceinline__ float expf_cw_core_scaled(float y, const ExpScaled& e, float c0) {
+  const float k = __builtin_rintf(y * e.log2e);
+  float r = __builtin_fmaf(-k, e.ln2_hi, y);
+  r = __builtin_fmaf(-k, e.ln2_lo, r);
+  float p = c0;
+  p = __builtin_fmaf(p, r, e.c[1]);
+  p = __builtin_fmaf(p, r, e.c[2]);
+  p = __builtin_fmaf(p, r, e.c[3]);
+  p = __builtin_fmaf(p, r, e.c[4]);
+  p = __builtin_fmaf(p, r, e.c[5]);
+  p = __builtin_fmaf(p, r, 1.0f);
+  return __builtin_ldexpf(p, static_cast<int>(k));
+}
+__device__ __forceinline__ float expf_cw_scaled(float y, const ExpScaled& es, float c0) {
+  float e = expf_cw_core_scaled(__builtin_fminf(__builtin_fmaxf(y, es.lo), es.hi), es, c0);
+  e = (y > es.hi) ? __builtin_inff() : e;
+  e = (y < es.lo) ? 0.0f : e;
+  return (y != y) ? y : e;
 }
 
 // Double precision: the same method with a 29-bit ln2_hi (k*ln2_hi exact for
@@ -143,12 +190,18 @@ __device__ __forceinline__ double expd_cw(double x) {
 // r + ulp, whose fma residuals s - r' * r carry the sign of the exact error
 // there (no denormal intermediates). Zero, tiny, negative and NaN inputs take
 // the IEEE library sqrt.
-__device__ __forceinline__ float sqrt_rn(float s) {
+// The test alone, for s >= 2^-96 or s == 0 (with s == 0, r - ulp is a NaN, whose
+// residual fails the test, and r + ulp's residual is +0: r = 0 stands).
+__device__ __forceinline__ float sqrt_rn_normal(float s) {
   const float r = __builtin_amdgcn_sqrtf(s);
   const float rm = __int_as_float(__float_as_int(r) - 1);
   const float rp = __int_as_float(__float_as_int(r) + 1);
   float q = (__builtin_fmaf(-rm, r, s) <= 0.0f) ? rm : r;
   q = (__builtin_fmaf(-rp, r, s) > 0.0f) ? rp : q;
+  return q;
+}
+__device__ __forceinline__ float sqrt_rn(float s) {
+  float q = sqrt_rn_normal(s);
   if (__builtin_expect(!(s >= 0x1p-96f), 0)) q = __builtin_sqrtf(s);
   return q;
 }

@@ -5,6 +5,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -24,6 +25,20 @@ using clue_dev::scan_temp_elems;
 
 constexpr int kChunk = 2048;  // points per chunk of the per-point grids
 constexpr int kDefaultPointsPerTile = 128;
+
+// Constants of exp(y * 2^-s) (clue_dev::ExpScaled): each constant of the float
+// exp scaled by a power of two, exact for |s| <= 8.
+inline clue_dev::ExpScaled exp_scaled(int s) {
+  clue_dev::ExpScaled e{};
+  e.log2e = std::ldexp(clue_dev::kExpLog2e, -s);
+  e.ln2_hi = std::ldexp(clue_dev::kExpLn2Hi, s);
+  e.ln2_lo = std::ldexp(clue_dev::kExpLn2Lo, s);
+  for (int i = 0; i < 5; ++i) e.c[i] = std::ldexp(clue_dev::kExpPoly[i], -s * (6 - i));
+  e.c[5] = std::ldexp(1.0f, -s);
+  e.lo = std::ldexp(clue_dev::kExpCoreLo, s);
+  e.hi = std::ldexp(clue_dev::kExpCoreHi, s);
+  return e;
+}
 
 // Capacity a grow-only device buffer holds after a request of `bytes`: unchanged
 // when it already fits, else the request plus a quarter and 256 bytes.

@@ -128,6 +128,12 @@ RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPl
     const T lim = sizeof(T) == 4 ? T(0x1p120) : T(0x1p1000);
     rp.gauss_pow2 = (m == T(0.5) && rp.gauss_den >= T(1) / lim && rp.gauss_den <= lim) ? 1 : 0;
     rp.gauss_inv_den = rp.gauss_pow2 ? std::ldexp(T(1), 1 - ex) : T(0);
+    // Float: fold gauss_den = 2^s into the exp's constants when |s| <= 8.
+    rp.gauss_fold = 0;
+    if (sizeof(T) == 4 && rp.gauss_pow2 && ex - 1 >= -8 && ex - 1 <= 8) {
+      rp.gauss_fold = 1;
+      rp.es = exp_scaled(ex - 1);
+    }
   }
   // Within the radius the kernel's exp argument is -lambda*d (exp) or -d*d/(2s^2)
   // (gauss), with 0 <= d <= dc; bound its magnitude with a 1% margin for rounding.

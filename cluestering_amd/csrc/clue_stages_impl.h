@@ -560,12 +560,18 @@ __device__ __forceinline__ T metric_value(const T (&xi)[D], const T (&si)[D],
   return metric_pair<D, M>(xi, si, xj, sj, rp);
 }
 
-template <int M, class T>
+// Normal: the caller has shown v >= 2^-96 or v == 0 (float sqrt without its
+// tiny-input branch).
+template <int M, bool Normal = false, class T>
 __device__ __forceinline__ T finish_distance(T v) {
-  if constexpr (MetricTraits<M>::kSqrt)
-    return sqrt_rn(v);
-  else
+  if constexpr (MetricTraits<M>::kSqrt) {
+    if constexpr (Normal && sizeof(T) == 4)
+      return sqrt_rn_normal(v);
+    else
+      return sqrt_rn(v);
+  } else {
     return v;
+  }
 }
 
 // Search box of one point (ClusteringKernels.hpp:111-121, TilesView.hpp:77-87).
@@ -606,8 +612,8 @@ __device__ __forceinline__ void walk_tiles(const int (&lo)[D], const int (&hi)[D
   int ulo[D], uhi[D], b[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    ulo[d] = uniform(wave_min_i(lo[d]));
-    uhi[d] = uniform(wave_max_i(hi[d]));
+    ulo[d] = wave_min_uniform(lo[d]);
+    uhi[d] = wave_max_uniform(hi[d]);
     b[d] = ulo[d];
   }
   if (ulo[0] > uhi[0]) return;  // no active lane
@@ -883,35 +889,45 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
 // Kernel values (ConvolutionalKernel.hpp:21-57) for a candidate within the radius.
 // K_GAUSS2 is the gaussian whose 2*sigma*sigma is a power of two: there division
 // by it and multiplication by its reciprocal are the same correctly rounded
-// operation on the same exact value, so the cheaper one is used. The _UNIT forms
-// are those of amplitude 1, whose product A * e is e itself (denormals are kept,
-// so even a subnormal e is unchanged).
+// operation on the same exact value, so the cheaper one is used (in float, the
+// power of two is folded into the exp's constants instead, ExpScaled). The _UNIT
+// forms are those of amplitude 1, whose product A * e is e itself (denormals are
+// kept, so even a subnormal e is unchanged).
 // ---------------------------------------------------------------------------
 enum { K_GAUSS2 = 3, K_EXP_UNIT = 4, K_GAUSS2_UNIT = 5 };
 
 // exp of an argument the host has shown to lie in [LO, HI] for every candidate
-// within the radius (rp.exp_in_range) skips the range handling; both forms give
-// the same value there.
-__device__ __forceinline__ float kernel_exp(float x, const RunParamsT<float>& rp) {
-  return rp.exp_in_range ? expf_cw_core(x) : expf_cw(x);
+// within the radius (rp.exp_in_range, passed as InRange) skips the range
+// handling; both forms give the same value there.
+template <bool InRange>
+__device__ __forceinline__ float kernel_exp(float x, const RunParamsT<float>&) {
+  return InRange ? expf_cw_core(x) : expf_cw(x);
 }
-__device__ __forceinline__ double kernel_exp(double x, const RunParamsT<double>& rp) {
-  return rp.exp_in_range ? expd_cw_core(x) : expd_cw(x);
+template <bool InRange>
+__device__ __forceinline__ double kernel_exp(double x, const RunParamsT<double>&) {
+  return InRange ? expd_cw_core(x) : expd_cw(x);
 }
 
-template <int M, int K, class T>
-__device__ __forceinline__ T kernel_value(T v, const RunParamsT<T>& rp) {
+// c0: rp.es.c[0] (float gaussian with a folded scale), possibly in a vector register.
+template <int M, int K, bool InRange, bool Normal, class T>
+__device__ __forceinline__ T kernel_value(T v, const RunParamsT<T>& rp, T c0) {
   if constexpr (K == K_FLAT) {
     return rp.kp0;
   } else if constexpr (K == K_EXP || K == K_EXP_UNIT) {
-    const T e = kernel_exp(-rp.kp0 * finish_distance<M>(v), rp);
+    const T e = kernel_exp<InRange>(-rp.kp0 * finish_distance<M, Normal>(v), rp);
     return K == K_EXP_UNIT ? e : rp.kp1 * e;
   } else if constexpr (K == K_GAUSS) {
-    const T dd = finish_distance<M>(v);
-    return rp.kp1 * kernel_exp(-dd * dd / rp.gauss_den, rp);
+    const T dd = finish_distance<M, Normal>(v);
+    return rp.kp1 * kernel_exp<InRange>(-dd * dd / rp.gauss_den, rp);
+  } else if constexpr (sizeof(T) == 4) {
+    // exp((-dd*dd) * 2^-s) with 2^-s = 1 / gauss_den folded into rp.es.
+    const T dd = finish_distance<M, Normal>(v);
+    const T y = -dd * dd;
+    const T e = InRange ? expf_cw_core_scaled(y, rp.es, c0) : expf_cw_scaled(y, rp.es, c0);
+    return K == K_GAUSS2_UNIT ? e : rp.kp1 * e;
   } else {
-    const T dd = finish_distance<M>(v);
-    const T e = kernel_exp((-dd * dd) * rp.gauss_inv_den, rp);
+    const T dd = finish_distance<M, Normal>(v);
+    const T e = kernel_exp<InRange>((-dd * dd) * rp.gauss_inv_den, rp);
     return K == K_GAUSS2_UNIT ? e : rp.kp1 * e;
   }
 }
@@ -924,6 +940,27 @@ __device__ __forceinline__ uint64_t density_key(float rho, uint32_t tag) {
 __device__ __forceinline__ uint64_t density_key(double rho, uint32_t) {
   return rho_key_f64(rho, false);
 }
+
+// Loads U consecutive wave-uniform column values (a block of candidates) with
+// one wide scalar load and holds them in scalar registers at this point: left to
+// itself the compiler may sink each value's load into the candidate's branch,
+// where every use then waits on its own load.
+template <int U, class T>
+__device__ __forceinline__ void load_block(const T* __restrict__ p, T (&out)[U]) {
+  typedef T Vec __attribute__((ext_vector_type(U), aligned(sizeof(T))));
+  Vec v = *reinterpret_cast<const Vec*>(p);
+  asm volatile("" : "+s"(v));
+#pragma unroll
+  for (int k = 0; k < U; ++k) out[k] = v[k];
+}
+
+// Compile-time loop modes of the density stage: kernel exp arguments known to be
+// in range, and square roots known to need no tiny-input branch.
+template <bool InRange, bool Normal>
+struct ModeC {
+  static constexpr bool kInRange = InRange;
+  static constexpr bool kNormal = Normal;
+};
 
 // Candidates are walked in blocks of U so that their wave-uniform data arrives
 // in a few wide scalar loads (measured on MI355X: 8 for density, 16 for
@@ -979,11 +1016,22 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
   // come pre-multiplied (fw = f * w).
   const T wself = active ? sw[s] : T(0);
   const T* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
+  // The folded gaussian's leading coefficient, held in a vector register for the
+  // whole walk (an fma takes one scalar operand; the next coefficient is one).
+  T c0 = T(0);
+  if constexpr (sizeof(T) == 4) {
+    c0 = rp.es.c[0];
+    asm volatile("" : "+v"(c0));
+  }
 
   // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
   // wj is the candidate's column value, loaded with its block of candidates.
-  auto add = [&](T v, int j, T wj, T thr, auto own_c) {
+  // own_c: std::true_type for the point's own tile, where the diagonal term lies;
+  // mode_c: ModeC<in range, normal> (see below).
+  auto add = [&](T v, int j, T wj, T thr, auto own_c, auto mode_c) {
     constexpr bool own = decltype(own_c)::value;
+    constexpr bool inr = decltype(mode_c)::kInRange;
+    constexpr bool nrm = decltype(mode_c)::kNormal;
     if constexpr (K == K_FLAT) {
       // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
       // the reference's addition of a zero term.
@@ -992,39 +1040,59 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
     } else if constexpr (K == K_EXP_UNIT || K == K_GAUSS2_UNIT) {
       // Amplitude 1: the diagonal's distance is 0 and exp(-0) is exactly 1, the
       // reference's K(i, i), so the diagonal needs no case of its own.
-      if (v <= thr) rho += kernel_value<M, K>(v, rp) * wj;
+      if (v <= thr) rho += kernel_value<M, K, inr, nrm>(v, rp, c0) * wj;
     } else {
       if (v <= thr) {
-        const T kv = (own && j == s) ? T(1) : kernel_value<M, K>(v, rp);
+        const T kv = (own && j == s) ? T(1) : kernel_value<M, K, inr, nrm>(v, rp, c0);
         rho += kv * wj;
       }
     }
   };
-  // own_c: std::true_type for the point's own tile, where the diagonal term lies.
-  auto walk = [&](int j0, int j1, T thr, auto own_c) {
+  auto walk = [&](int j0, int j1, T thr, auto own_c, auto mode_c) {
     int j = j0;
     for (; j + U <= j1; j += U) {
       T v[U], w[U];
 #pragma unroll
       for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
+      load_block<U>(wcol + j, w);
 #pragma unroll
-      for (int k = 0; k < U; ++k) w[k] = wcol[j + k];
-#pragma unroll
-      for (int k = 0; k < U; ++k) add(v[k], j + k, w[k], thr, own_c);
+      for (int k = 0; k < U; ++k) add(v[k], j + k, w[k], thr, own_c, mode_c);
     }
     for (; j < j1; ++j)
-      add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c);
+      add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c, mode_c);
   };
-
-  walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
-    const int j0 = toff[t];
-    const int j1 = toff[t + 1];
-    const T thr = in ? thr_in : T(-1);
-    if (t == it.tile)
-      walk(j0, j1, thr, std::true_type{});
+  auto run = [&](auto mode_c) {
+    walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+      const int j0 = toff[t];
+      const int j1 = toff[t + 1];
+      const T thr = in ? thr_in : T(-1);
+      if (t == it.tile)
+        walk(j0, j1, thr, std::true_type{}, mode_c);
+      else
+        walk(j0, j1, thr, std::false_type{}, mode_c);
+    });
+  };
+  // Both decisions are made once, outside the candidate loop:
+  //  - in range: launch-wide, every exp argument within the radius is in range;
+  //  - normal: Euclidean in float, and no lane of the wave has a coordinate of
+  //    magnitude below 2^-24. Then every pair with v > 0 has some |dx| >= 2^-48
+  //    (floats of magnitude >= 2^-24 are multiples of 2^-47, and the largest float
+  //    below 2^-24 is 2^-24 - 2^-48), so v >= 2^-96 and the square root needs no
+  //    tiny-input branch.
+  if constexpr (K == K_FLAT) {
+    run(ModeC<true, false>{});
+  } else {
+    bool tiny = false;
+#pragma unroll
+    for (int d = 0; d < D; ++d) tiny = tiny || (__builtin_fabs(xi[d]) < T(0x1p-24));
+    const bool normal = (M == M_EUCLID) && sizeof(T) == 4 && ballot(active && tiny) == 0ull;
+    if (rp.exp_in_range && normal)
+      run(ModeC<true, true>{});
+    else if (rp.exp_in_range)
+      run(ModeC<true, false>{});
     else
-      walk(j0, j1, thr, std::false_type{});
-  });
+      run(ModeC<false, false>{});
+  }
   if (active) {
     srho[s] = rho;
     skey[s] = density_key(rho, stag[s]);
@@ -1293,7 +1361,9 @@ __device__ __forceinline__ T metric_lb(const T (&qlo)[D], const T (&qhi)[D], con
   }
 }
 
-// Wave-wide minimum and maximum of floating-point values and of unsigned keys.
+// Wave-wide minimum and maximum of floating-point values, and a lower bound of
+// the wave's smallest ordering key. (LDS permutes here: the DPP form costs more
+// vector instructions, and this scan is short of them; measured on MI355X.)
 template <class T>
 __device__ __forceinline__ T wave_min_f(T v) {
 #pragma unroll
@@ -1306,14 +1376,18 @@ __device__ __forceinline__ T wave_max_f(T v) {
   for (int off = 32; off > 0; off >>= 1) v = fmax_t(v, __shfl_xor(v, off, 64));
   return v;
 }
-__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+// The smallest high word with a zero low word: never above the smallest key, so
+// "a block key above it" keeps every block that could hold a higher candidate.
+__device__ __forceinline__ uint64_t wave_min_key_bound(uint64_t v) {
+  uint32_t h = static_cast<uint32_t>(v >> 32);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = shfl_xor_u64(v, off);
-    v = o < v ? o : v;
+    const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(h), off, 64));
+    h = o < h ? o : h;
   }
-  return v;
+  return static_cast<uint64_t>(h) << 32;
 }
+
 // Lane i's value in every lane (i wave-uniform).
 __device__ __forceinline__ float read_lane(float v, int i) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), i));
@@ -1486,7 +1560,7 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
       qlo[d] = wave_min_f(in ? xi[d] : std::numeric_limits<T>::infinity());
       qhi[d] = wave_max_f(in ? xi[d] : -std::numeric_limits<T>::infinity());
     }
-    const uint64_t kmin = wave_min_u64(in ? key_i : ~0ull);
+    const uint64_t kmin = wave_min_key_bound(in ? key_i : ~0ull);
     for (int c0 = 0; c0 < nblk; c0 += kWave) {
       const T bmax = wave_max_f(in ? bound : -std::numeric_limits<T>::infinity());
       const int l = lane_id();
@@ -1768,14 +1842,17 @@ static void launch_density_dm(const NeighbourLaunch<T>& L) {
       else
         launch_density_dmk<D, M, K_EXP>(L);
       break;
-    default:
-      if (L.rp->gauss_pow2 && L.rp->kp1 == T(1))
+    default: {
+      // Float folds the power of two into the exp's constants (|s| <= 8 only).
+      const bool pow2 = sizeof(T) == 4 ? L.rp->gauss_fold != 0 : L.rp->gauss_pow2 != 0;
+      if (pow2 && L.rp->kp1 == T(1))
         launch_density_dmk<D, M, K_GAUSS2_UNIT>(L);
-      else if (L.rp->gauss_pow2)
+      else if (pow2)
         launch_density_dmk<D, M, K_GAUSS2>(L);
       else
         launch_density_dmk<D, M, K_GAUSS>(L);
       break;
+    }
   }
 }
 
@@ -1823,8 +1900,10 @@ static void launch_nh_blocks_d(const NeighbourLaunch<T>& L) {
   }
 }
 
+// Defined here, instantiated per dimension in clue_neighbour.hip (one object per
+// dimension, compiled in parallel); the pipeline's translation units only call it.
 template <int D, class T>
-static void launch_neighbour_d(const NeighbourLaunch<T>& L, bool density) {
+void launch_neighbour_d(const NeighbourLaunch<T>& L, bool density) {
   if (!density && L.rp->nh_blocks && L.rp->metric != M_MAHALANOBIS) {
     launch_nh_blocks_d<D>(L);
     return;
@@ -1851,6 +1930,21 @@ static void launch_neighbour_d(const NeighbourLaunch<T>& L, bool density) {
   }
 #undef CLUE_METRIC_CASE
 }
+
+#define CLUE_NEIGHBOUR_EXTERN(D)                                                          \
+  extern template void launch_neighbour_d<D, float>(const NeighbourLaunch<float>&, bool); \
+  extern template void launch_neighbour_d<D, double>(const NeighbourLaunch<double>&, bool);
+CLUE_NEIGHBOUR_EXTERN(1)
+CLUE_NEIGHBOUR_EXTERN(2)
+CLUE_NEIGHBOUR_EXTERN(3)
+CLUE_NEIGHBOUR_EXTERN(4)
+CLUE_NEIGHBOUR_EXTERN(5)
+CLUE_NEIGHBOUR_EXTERN(6)
+CLUE_NEIGHBOUR_EXTERN(7)
+CLUE_NEIGHBOUR_EXTERN(8)
+CLUE_NEIGHBOUR_EXTERN(9)
+CLUE_NEIGHBOUR_EXTERN(10)
+#undef CLUE_NEIGHBOUR_EXTERN
 
 template <int D, class T>
 static void launch_query_order_d(const NeighbourLaunch<T>& L) {

@@ -27,6 +27,31 @@ inline int64_t radix_hist_elems(int64_t n) {
   return kRsDigitBins * ((n + kRsTileKeys - 1) / kRsTileKeys);
 }
 
+// Constants of the float exp (Cody-Waite reduction x = k ln2 + r with a 15-bit
+// ln2_hi, then the degree-6 minimax polynomial of e^r on [-ln2/2, ln2/2] whose
+// constant and linear terms are 1), shared by the device code and the host code
+// that folds power-of-two argument scales into them.
+constexpr float kExpLog2e = 0x1.715476p+0f;
+constexpr float kExpLn2Hi = 0x1.62e400p-1f;
+constexpr float kExpLn2Lo = 0x1.7f7d1cp-20f;
+constexpr float kExpPoly[5] = {0x1.6a244cp-10f, 0x1.1239d4p-7f, 0x1.5558f2p-5f, 0x1.555492p-3f,
+                               0x1.fffffcp-2f};  // c6, c5, c4, c3, c2
+constexpr float kExpCoreLo = -0x1.9fe368p+6f;  // the core needs no range handling on [lo, hi]
+constexpr float kExpCoreHi = 0x1.62e42ep+6f;
+
+// exp(y * 2^-s) evaluated without forming y * 2^-s: every step of the reduction
+// and of the Horner polynomial is the float exp's own step scaled by a power of
+// two (k = rint(y * (log2e 2^-s)); r' = 2^s r from ln2 2^s; the Horner partial
+// sums carry 2^(-s i) through coefficients c_i 2^(-s i)), so with |s| <= 8 no
+// step leaves the normal range and the bits equal exp(rn(y * 2^-s)) for every y.
+struct ExpScaled {
+  float log2e;   // log2e * 2^-s
+  float ln2_hi;  // ln2_hi * 2^s
+  float ln2_lo;  // ln2_lo * 2^s
+  float c[6];    // c6 .. c1, c_i * 2^(-s i)
+  float lo, hi;  // the core range, times 2^s
+};
+
 // One event of a batch. All offsets are host-computed from the event sizes.
 struct EventDesc {
   int32_t p_off;   // first point of the event in the packed arrays
@@ -78,7 +103,9 @@ struct RunParamsT {
   T gauss_den;           // (2 * sigma) * sigma, as the reference evaluates it
   T gauss_inv_den;       // 1 / gauss_den when gauss_den is a power of two
   int32_t gauss_pow2;    // gauss_den is a power of two
+  int32_t gauss_fold;    // float runs: gauss_den = 2^s with |s| <= 8, folded into es
   int32_t exp_in_range;  // every kernel exp argument within the radius is in the core range
+  ExpScaled es;          // float runs: exp(y * 2^-s) with 2^s = gauss_den
   int32_t wrapped[kMaxDim];
   T mparam[kMaxDim];
   int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none

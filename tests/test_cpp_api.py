@@ -37,6 +37,14 @@ def test_oracle_under_sanitizers():
     _run("test_oracle_sanitized")
 
 
+def test_folded_gaussian_exp_equals_the_oracle():
+    """exp((-d*d) * 2^-s) with the power of two folded into the exp's constants, as
+    the float density stage evaluates the gaussian with 2*sigma*sigma = 2^s, gives
+    the oracle's bits (every y of the core range for s = 1, a stride of all floats
+    for |s| <= 8; `test_exp_scaled --exhaustive` widens both)."""
+    _run("test_exp_scaled")
+
+
 @pytest.mark.gpu
 def test_cpp_dropin_program():
     _run("test_clusterer", os.path.join(ROOT, "tests", "golden", "reference_data"))
