@@ -105,7 +105,8 @@ class Tuning(C.Structure):
         ("xcd_chunk_log2", C.c_int32),
         ("block_waves", C.c_int32),
         ("nh_plain_scan", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("persistent", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

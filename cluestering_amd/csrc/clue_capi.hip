@@ -541,6 +541,8 @@ clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning) {
       tuning->block_waves != 2 && tuning->block_waves != 4 && tuning->block_waves != 8 &&
       tuning->block_waves != 16)
     return fail(CLUE_ERR_INVALID_ARGUMENT, "block_waves must be 1, 2, 4, 8 or 16");
+  if (tuning && (tuning->persistent < 0 || tuning->persistent > 64))
+    return fail(CLUE_ERR_INVALID_ARGUMENT, "persistent must lie in [0, 64] (waves per CU)");
   ctx->tuned = tuning != nullptr;
   if (tuning) {
     ctx->tuning = clue_tuning{};
@@ -548,6 +550,7 @@ clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning) {
     ctx->tuning.xcd_chunk_log2 = tuning->xcd_chunk_log2;
     ctx->tuning.block_waves = tuning->block_waves ? tuning->block_waves : 4;
     ctx->tuning.nh_plain_scan = tuning->nh_plain_scan ? 1 : 0;
+    ctx->tuning.persistent = tuning->persistent;
   }
   return CLUE_OK;
 }

@@ -90,6 +90,7 @@ inline clue_tuning table_tuning(int ndim, int64_t n_total, int64_t n_events) {
       out.group_queries = t.group_queries;
       out.xcd_chunk_log2 = t.xcd_chunk_log2;
       out.block_waves = t.block_waves;
+      out.persistent = t.persistent;
       return out;
     }
   out.group_queries = 1;
@@ -204,7 +205,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_SCAN] = sizeof(int32_t) * scan_elems;
   bytes[S_TOFF] = sizeof(int32_t) * (NT + 1);
   bytes[S_ICOUNT] = sizeof(int32_t) * NT;
-  bytes[S_NITEMS] = sizeof(int32_t) * 4;
+  bytes[S_NITEMS] = sizeof(int32_t) * 32;  // item count, then the work-item queue counters
   bytes[S_ITEMS] = sizeof(Item) * w.max_items;
   bytes[S_QPERM] = w.group_queries ? sizeof(int32_t) * n : 0;
   bytes[S_QPERM_NH] = (w.group_queries && w.separate_dm_order) ? sizeof(int32_t) * n : 0;

@@ -101,6 +101,7 @@ RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPl
   RunParamsT<T> rp{};
   rp.xcd_chunk_log2 = bp.tuning.xcd_chunk_log2;
   rp.block_waves = bp.tuning.block_waves > 0 ? bp.tuning.block_waves : 4;
+  rp.persistent = bp.tuning.persistent > 0 ? bp.tuning.persistent : 0;
   rp.nh_blocks = nh_blocks_enabled(p, pts, bp) ? 1 : 0;
   rp.D = p->ndim;
   rp.kernel = p->kernel;

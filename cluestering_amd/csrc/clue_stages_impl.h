@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <limits>
 #include <type_traits>
@@ -59,6 +60,40 @@ __device__ __forceinline__ int xcd_block(int b, int nb, int log2_chunk) {
   const int k = b / kXcds;  // b is the k-th block on its XCD
   return ((((k >> log2_chunk) * kXcds) + xcd) << log2_chunk) + (k & ((1 << log2_chunk) - 1));
 }
+
+// Work items of a neighbour stage. In the grid form each wave takes one item (its
+// workgroup mapped by xcd_block). In the persistent form (rp.persistent waves per
+// CU) waves pull items from device queues: the item space is dealt to the XCDs in
+// chunks of 2^k items, as xcd_block deals workgroups, each XCD counts off its own
+// chunks (ctl[x], zeroed by k_item_fill), and a wave whose XCD has no items left
+// takes them from the next XCDs in turn. A queue's items come in increasing order,
+// so a wave stops once every one of the eight counters has passed the last item.
+constexpr int kQueueDensity = 8;  // counters in the n_items block: density
+constexpr int kQueueNh = 16;      // nearest-higher
+constexpr int kQueueWords = 24;   // words of the block k_item_fill zeroes
+struct ItemCursor {
+  int32_t* ctl;
+  int n;
+  int log2c;
+  int home;
+
+  template <class P>
+  __device__ ItemCursor(int32_t* n_items, int queue, const P& rp)
+      : ctl(n_items + queue), n(*n_items), log2c(rp.xcd_chunk_log2), home(blockIdx.x % kXcds) {}
+
+  __device__ int pull() {
+    for (int t = 0; t < kXcds; ++t) {
+      const int x = (home + t) & (kXcds - 1);
+      int j = 0;
+      if (lane_id() == 0) j = atomicAdd(&ctl[x], 1);
+      j = uniform(j);
+      const int item = log2c < 0 ? j * kXcds + x
+                                 : ((x + kXcds * (j >> log2c)) << log2c) + (j & ((1 << log2c) - 1));
+      if (item < n) return item;
+    }
+    return -1;
+  }
+};
 
 template <int M>
 struct MetricTraits {
@@ -453,8 +488,9 @@ static __global__ void k_item_count(const int32_t* __restrict__ toff, int n_tile
 
 static __global__ void k_item_fill(const int32_t* __restrict__ toff, const int32_t* __restrict__ ioff,
                             int n_tiles, const EventDesc* __restrict__ events, int n_events,
-                            Item* __restrict__ items) {
+                            Item* __restrict__ items, int32_t* __restrict__ queues) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= kQueueDensity && k < kQueueWords) queues[k] = 0;  // work-item queue counters
   if (k >= n_tiles) return;
   // Event of tile k: last event with t_base <= k.
   int lo = 0, hi = n_events - 1;
@@ -976,126 +1012,133 @@ struct Unroll {
 // (ClusteringKernels.hpp:53-69). Candidates beyond dc add +0 and are skipped;
 // the additions keep the candidate order.
 // ---------------------------------------------------------------------------
-template <int D, int M, int K, class T>
+template <int D, int M, int K, bool Persistent, class T>
 __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
-    const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
+    int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
     const T* __restrict__ sw, const T* __restrict__ sfw, const T* __restrict__ ssig,
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
     T* __restrict__ srho, uint64_t* __restrict__ skey) {
   constexpr int U = Unroll<D>::U;
-  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
-                   threadIdx.x / kWave;
-  if (item >= *n_items) return;
-  const Item it = items[item];
-  const EventDesc ev = events[it.event];
-  const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
-  const int q = it.begin + lane_id();
-  const bool active = q < it.end;
-  const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
+  auto process = [&](const int item) {
+    const Item it = items[item];
+    const EventDesc ev = events[it.event];
+    const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
+    const int q = it.begin + lane_id();
+    const bool active = q < it.end;
+    const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
-  T xi[D], si[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    xi[d] = active ? sx[d * cs + s] : T(0);
-    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
-  }
-  int lo[D], hi[D];
-  search_box<D>(g, xi, si, rp.dc, lo, hi);
-  if (!active) {
-#pragma unroll
+    T xi[D], si[D];
+  #pragma unroll
     for (int d = 0; d < D; ++d) {
-      lo[d] = INT32_MAX;
-      hi[d] = INT32_MIN;
+      xi[d] = active ? sx[d * cs + s] : T(0);
+      si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
     }
-  }
-  const T thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
-  T rho = T(0);
-  // The diagonal term K(i,i) * w_i = w_i; the flat kernel's off-diagonal weights
-  // come pre-multiplied (fw = f * w).
-  const T wself = active ? sw[s] : T(0);
-  const T* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
-  // The folded gaussian's leading coefficient, held in a vector register for the
-  // whole walk (an fma takes one scalar operand; the next coefficient is one).
-  T c0 = T(0);
-  if constexpr (sizeof(T) == 4) {
-    c0 = rp.es.c[0];
-    asm volatile("" : "+v"(c0));
-  }
-
-  // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
-  // wj is the candidate's column value, loaded with its block of candidates.
-  // own_c: std::true_type for the point's own tile, where the diagonal term lies;
-  // mode_c: ModeC<in range, normal> (see below).
-  auto add = [&](T v, int j, T wj, T thr, auto own_c, auto mode_c) {
-    constexpr bool own = decltype(own_c)::value;
-    constexpr bool inr = decltype(mode_c)::kInRange;
-    constexpr bool nrm = decltype(mode_c)::kNormal;
-    if constexpr (K == K_FLAT) {
-      // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
-      // the reference's addition of a zero term.
-      const T next = rho + ((own && j == s) ? wself : wj);
-      rho = (v <= thr) ? next : rho;
-    } else if constexpr (K == K_EXP_UNIT || K == K_GAUSS2_UNIT) {
-      // Amplitude 1: the diagonal's distance is 0 and exp(-0) is exactly 1, the
-      // reference's K(i, i), so the diagonal needs no case of its own.
-      if (v <= thr) rho += kernel_value<M, K, inr, nrm>(v, rp, c0) * wj;
-    } else {
-      if (v <= thr) {
-        const T kv = (own && j == s) ? T(1) : kernel_value<M, K, inr, nrm>(v, rp, c0);
-        rho += kv * wj;
+    int lo[D], hi[D];
+    search_box<D>(g, xi, si, rp.dc, lo, hi);
+    if (!active) {
+  #pragma unroll
+      for (int d = 0; d < D; ++d) {
+        lo[d] = INT32_MAX;
+        hi[d] = INT32_MIN;
       }
     }
-  };
-  auto walk = [&](int j0, int j1, T thr, auto own_c, auto mode_c) {
-    int j = j0;
-    for (; j + U <= j1; j += U) {
-      T v[U], w[U];
-#pragma unroll
-      for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
-      load_block<U>(wcol + j, w);
-#pragma unroll
-      for (int k = 0; k < U; ++k) add(v[k], j + k, w[k], thr, own_c, mode_c);
+    const T thr_in = MetricTraits<M>::kSqrt ? rp.dc_sq_max : rp.dc;
+    T rho = T(0);
+    // The diagonal term K(i,i) * w_i = w_i; the flat kernel's off-diagonal weights
+    // come pre-multiplied (fw = f * w).
+    const T wself = active ? sw[s] : T(0);
+    const T* __restrict__ wcol = (K == K_FLAT) ? sfw : sw;
+    // The folded gaussian's leading coefficient, held in a vector register for the
+    // whole walk (an fma takes one scalar operand; the next coefficient is one).
+    T c0 = T(0);
+    if constexpr (sizeof(T) == 4) {
+      c0 = rp.es.c[0];
+      asm volatile("" : "+v"(c0));
     }
-    for (; j < j1; ++j)
-      add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c, mode_c);
-  };
-  auto run = [&](auto mode_c) {
-    walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
-      const int j0 = toff[t];
-      const int j1 = toff[t + 1];
-      const T thr = in ? thr_in : T(-1);
-      if (t == it.tile)
-        walk(j0, j1, thr, std::true_type{}, mode_c);
-      else
-        walk(j0, j1, thr, std::false_type{}, mode_c);
-    });
-  };
-  // Both decisions are made once, outside the candidate loop:
-  //  - in range: launch-wide, every exp argument within the radius is in range;
-  //  - normal: Euclidean in float, and no lane of the wave has a coordinate of
-  //    magnitude below 2^-24. Then every pair with v > 0 has some |dx| >= 2^-48
-  //    (floats of magnitude >= 2^-24 are multiples of 2^-47, and the largest float
-  //    below 2^-24 is 2^-24 - 2^-48), so v >= 2^-96 and the square root needs no
-  //    tiny-input branch.
-  if constexpr (K == K_FLAT) {
-    run(ModeC<true, false>{});
-  } else {
-    bool tiny = false;
-#pragma unroll
-    for (int d = 0; d < D; ++d) tiny = tiny || (__builtin_fabs(xi[d]) < T(0x1p-24));
-    const bool normal = (M == M_EUCLID) && sizeof(T) == 4 && ballot(active && tiny) == 0ull;
-    if (rp.exp_in_range && normal)
-      run(ModeC<true, true>{});
-    else if (rp.exp_in_range)
+
+    // term(j) = K(d_ij) * w_j, with K = 1 on the diagonal (self only in its own tile).
+    // wj is the candidate's column value, loaded with its block of candidates.
+    // own_c: std::true_type for the point's own tile, where the diagonal term lies;
+    // mode_c: ModeC<in range, normal> (see below).
+    auto add = [&](T v, int j, T wj, T thr, auto own_c, auto mode_c) {
+      constexpr bool own = decltype(own_c)::value;
+      constexpr bool inr = decltype(mode_c)::kInRange;
+      constexpr bool nrm = decltype(mode_c)::kNormal;
+      if constexpr (K == K_FLAT) {
+        // rho + 0 == rho here (rho is never -0), so selecting the unchanged sum is
+        // the reference's addition of a zero term.
+        const T next = rho + ((own && j == s) ? wself : wj);
+        rho = (v <= thr) ? next : rho;
+      } else if constexpr (K == K_EXP_UNIT || K == K_GAUSS2_UNIT) {
+        // Amplitude 1: the diagonal's distance is 0 and exp(-0) is exactly 1, the
+        // reference's K(i, i), so the diagonal needs no case of its own.
+        if (v <= thr) rho += kernel_value<M, K, inr, nrm>(v, rp, c0) * wj;
+      } else {
+        if (v <= thr) {
+          const T kv = (own && j == s) ? T(1) : kernel_value<M, K, inr, nrm>(v, rp, c0);
+          rho += kv * wj;
+        }
+      }
+    };
+    auto walk = [&](int j0, int j1, T thr, auto own_c, auto mode_c) {
+      int j = j0;
+      for (; j + U <= j1; j += U) {
+        T v[U], w[U];
+  #pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = metric_value<D, M>(xi, si, sx, cs, j + k, ssig, rp);
+        load_block<U>(wcol + j, w);
+  #pragma unroll
+        for (int k = 0; k < U; ++k) add(v[k], j + k, w[k], thr, own_c, mode_c);
+      }
+      for (; j < j1; ++j)
+        add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c, mode_c);
+    };
+    auto run = [&](auto mode_c) {
+      walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+        const int j0 = toff[t];
+        const int j1 = toff[t + 1];
+        const T thr = in ? thr_in : T(-1);
+        if (t == it.tile)
+          walk(j0, j1, thr, std::true_type{}, mode_c);
+        else
+          walk(j0, j1, thr, std::false_type{}, mode_c);
+      });
+    };
+    // Both decisions are made once, outside the candidate loop:
+    //  - in range: launch-wide, every exp argument within the radius is in range;
+    //  - normal: Euclidean in float, and no lane of the wave has a coordinate of
+    //    magnitude below 2^-24. Then every pair with v > 0 has some |dx| >= 2^-48
+    //    (floats of magnitude >= 2^-24 are multiples of 2^-47, and the largest float
+    //    below 2^-24 is 2^-24 - 2^-48), so v >= 2^-96 and the square root needs no
+    //    tiny-input branch.
+    if constexpr (K == K_FLAT) {
       run(ModeC<true, false>{});
-    else
-      run(ModeC<false, false>{});
-  }
-  if (active) {
-    srho[s] = rho;
-    skey[s] = density_key(rho, stag[s]);
+    } else {
+      bool tiny = false;
+  #pragma unroll
+      for (int d = 0; d < D; ++d) tiny = tiny || (__builtin_fabs(xi[d]) < T(0x1p-24));
+      const bool normal = (M == M_EUCLID) && sizeof(T) == 4 && ballot(active && tiny) == 0ull;
+      if (rp.exp_in_range && normal)
+        run(ModeC<true, true>{});
+      else if (rp.exp_in_range)
+        run(ModeC<true, false>{});
+      else
+        run(ModeC<false, false>{});
+    }
+    if (active) {
+      srho[s] = rho;
+      skey[s] = density_key(rho, stag[s]);
+    }
+  };
+  if constexpr (Persistent) {
+    ItemCursor cursor(n_items, kQueueDensity, rp);
+    for (int item = cursor.pull(); item >= 0; item = cursor.pull()) process(item);
+  } else {
+    const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
+                     threadIdx.x / kWave;
+    if (item < *n_items) process(item);
   }
 }
 
@@ -1433,12 +1476,12 @@ __device__ __forceinline__ bool key_maybe_higher(uint64_t kj, uint64_t ki, doubl
 // this bound, 10.2 ms with it and register blocks of 8 candidates instead of
 // 16). From 3-D on the bound would spill to scratch; there the compiler chooses
 // (equal time on the 2^20-point 3-D event).
-template <int D, int M, class T>
+template <int D, int M, bool Persistent, class T>
 __global__ __launch_bounds__(kMaxNeighbourBlock)
 __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, const T* __restrict__ geo,
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
-    const int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
+    int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
     const T* __restrict__ ssig, const T* __restrict__ su, const uint32_t* __restrict__ stag,
     const T* __restrict__ srho, const uint64_t* __restrict__ skey,
     const int32_t* __restrict__ qperm, const T* __restrict__ hx, int64_t hs,
@@ -1446,178 +1489,185 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
     const T* __restrict__ hbox, const uint64_t* __restrict__ hbkey, int32_t* __restrict__ snh,
     T* __restrict__ sdelta) {
   constexpr int V = nh_sub_block<D>();
-  const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
-                   threadIdx.x / kWave;
-  if (item >= *n_items) return;
-  const Item it = items[item];
-  const EventDesc ev = events[it.event];
-  const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
-  const int q = it.begin + lane_id();
-  const bool active = q < it.end;
-  const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
+  auto process = [&](const int item) {
+    const Item it = items[item];
+    const EventDesc ev = events[it.event];
+    const GeoT<T> g = event_geo(geo, rp, it.event, ev.nper);
+    const int q = it.begin + lane_id();
+    const bool active = q < it.end;
+    const int s = active ? (qperm ? qperm[q] : q) : -1;  // sorted position of this lane's point
 
-  T xi[D], si[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    xi[d] = active ? sx[d * cs + s] : T(0);
-    si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
-  }
-  const T rho_i = active ? srho[s] : T(0);
-  const uint32_t tag_i = active ? stag[s] : 0u;
-  const uint64_t key_i = active ? query_key(rho_i, tag_i) : ~0ull;
-  const T ui = (active && rp.has_u) ? su[s] : T(1);
-  const T eff_min = rp.rhoc * ui;
-  const bool use_seed = rho_i >= eff_min;
-  const T eff = use_seed ? rp.seed_dc : rp.dm;
-  const T eff_thr =
-      MetricTraits<M>::kSqrt ? (use_seed ? rp.seed_sq_max : rp.dm_sq_max) : eff;
-
-  int lo[D], hi[D];
-  search_box<D>(g, xi, si, rp.dm, lo, hi);
-  if (!active) {
-#pragma unroll
+    T xi[D], si[D];
+  #pragma unroll
     for (int d = 0; d < D; ++d) {
-      lo[d] = INT32_MAX;
-      hi[d] = INT32_MIN;
+      xi[d] = active ? sx[d * cs + s] : T(0);
+      si[d] = (active && rp.has_sigma) ? ssig[d * cs + s] : T(0);
     }
-  }
-  T delta = std::numeric_limits<T>::max();
-  int64_t nh_slot = -1;  // layout slot of the nearest-higher found so far
-  T rho_nh = T(0);
-  uint32_t tag_nh = 0u;
-  T bound = eff_thr;  // prune_threshold(eff_thr, delta)
+    const T rho_i = active ? srho[s] : T(0);
+    const uint32_t tag_i = active ? stag[s] : 0u;
+    const uint64_t key_i = active ? query_key(rho_i, tag_i) : ~0ull;
+    const T ui = (active && rp.has_u) ? su[s] : T(1);
+    const T eff_min = rp.rhoc * ui;
+    const bool use_seed = rho_i >= eff_min;
+    const T eff = use_seed ? rp.seed_dc : rp.dm;
+    const T eff_thr =
+        MetricTraits<M>::kSqrt ? (use_seed ? rp.seed_sq_max : rp.dm_sq_max) : eff;
 
-  // The reference's acceptance test (ClusteringKernels.hpp:180-195).
-  auto consider = [&](T v, T rj, uint32_t tj, int64_t slot) {
-    const T dist = finish_distance<M>(v);
-    if (dist <= eff &&
-        ((dist < delta) ||
-         ((dist == delta) && (nh_slot >= 0) &&
-          ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
-      delta = dist;
-      nh_slot = slot;
-      rho_nh = rj;
-      tag_nh = tj;
-      bound = prune_threshold<M>(eff_thr, delta);
-    }
-  };
-
-  // One block's candidates, V at a time.
-  auto scan_block = [&](int64_t p0, bool in) {
-    for (int k0 = 0; k0 < kNhBlock; k0 += V) {
-      const int64_t p = p0 + k0;
-      T v[V];
-      T m = std::numeric_limits<T>::infinity();
-#pragma unroll
-      for (int k = 0; k < V; ++k) {
-        T xj[D], sj[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          xj[d] = hx[d * hs + p + k];
-          sj[d] = T(0);
-        }
-        v[k] = metric_pair<D, M>(xi, si, xj, sj, rp);
-        const bool h = in & key_maybe_higher(hkey[p + k], key_i, T(0));
-        m = fmin_t(m, h ? v[k] : std::numeric_limits<T>::infinity());
+    int lo[D], hi[D];
+    search_box<D>(g, xi, si, rp.dm, lo, hi);
+    if (!active) {
+  #pragma unroll
+      for (int d = 0; d < D; ++d) {
+        lo[d] = INT32_MAX;
+        hi[d] = INT32_MIN;
       }
-      if (ballot(m <= bound) != 0ull) {
-#pragma unroll
+    }
+    T delta = std::numeric_limits<T>::max();
+    int64_t nh_slot = -1;  // layout slot of the nearest-higher found so far
+    T rho_nh = T(0);
+    uint32_t tag_nh = 0u;
+    T bound = eff_thr;  // prune_threshold(eff_thr, delta)
+
+    // The reference's acceptance test (ClusteringKernels.hpp:180-195).
+    auto consider = [&](T v, T rj, uint32_t tj, int64_t slot) {
+      const T dist = finish_distance<M>(v);
+      if (dist <= eff &&
+          ((dist < delta) ||
+           ((dist == delta) && (nh_slot >= 0) &&
+            ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+        delta = dist;
+        nh_slot = slot;
+        rho_nh = rj;
+        tag_nh = tj;
+        bound = prune_threshold<M>(eff_thr, delta);
+      }
+    };
+
+    // One block's candidates, V at a time.
+    auto scan_block = [&](int64_t p0, bool in) {
+      for (int k0 = 0; k0 < kNhBlock; k0 += V) {
+        const int64_t p = p0 + k0;
+        T v[V];
+        T m = std::numeric_limits<T>::infinity();
+  #pragma unroll
         for (int k = 0; k < V; ++k) {
-          const uint64_t kj = hkey[p + k];
-          if (in & key_maybe_higher(kj, key_i, T(0)) & (v[k] <= bound)) {
-            if constexpr (sizeof(T) == 4) {
-              // Float keys are exact and carry rho, and the tag when rho > 0: no
-              // dependent loads on this path.
-              const float rj = key_rho(kj);
-              const uint32_t tj = rj > 0.0f ? static_cast<uint32_t>(kj) : stag[hidx[p + k]];
-              consider(v[k], rj, tj, p + k);
-            } else {
-              const int j = hidx[p + k];
-              const uint32_t tj = stag[j];
-              if (key_higher(kj, tj, key_i, tag_i, T(0))) consider(v[k], srho[j], tj, p + k);
+          T xj[D], sj[D];
+  #pragma unroll
+          for (int d = 0; d < D; ++d) {
+            xj[d] = hx[d * hs + p + k];
+            sj[d] = T(0);
+          }
+          v[k] = metric_pair<D, M>(xi, si, xj, sj, rp);
+          const bool h = in & key_maybe_higher(hkey[p + k], key_i, T(0));
+          m = fmin_t(m, h ? v[k] : std::numeric_limits<T>::infinity());
+        }
+        if (ballot(m <= bound) != 0ull) {
+  #pragma unroll
+          for (int k = 0; k < V; ++k) {
+            const uint64_t kj = hkey[p + k];
+            if (in & key_maybe_higher(kj, key_i, T(0)) & (v[k] <= bound)) {
+              if constexpr (sizeof(T) == 4) {
+                // Float keys are exact and carry rho, and the tag when rho > 0: no
+                // dependent loads on this path.
+                const float rj = key_rho(kj);
+                const uint32_t tj = rj > 0.0f ? static_cast<uint32_t>(kj) : stag[hidx[p + k]];
+                consider(v[k], rj, tj, p + k);
+              } else {
+                const int j = hidx[p + k];
+                const uint32_t tj = stag[j];
+                if (key_higher(kj, tj, key_i, tag_i, T(0))) consider(v[k], srho[j], tj, p + k);
+              }
             }
           }
         }
       }
-    }
-  };
+    };
 
-  // in: the lane's search box holds tile t. The boxes and keys of up to 64 blocks
-  // arrive in one vector load (one block per lane); a wave-wide test against the
-  // box, smallest key and largest bound of the lanes that hold the tile drops
-  // most blocks at once, and each block left is tested per lane before its scan.
-  // The tile's blocks are visited from block sb on, wrapping around: in the own
-  // tile sb is the block that holds the wave's first point, whose neighbours
-  // settle most bounds before the rest of the tile is tested. The result does
-  // not depend on the order (the acceptance test is a strict total order).
-  auto scan_tile = [&](int t, bool in, int sb) {
-    const int b = toff[t];
-    const int nblk = (toff[t + 1] - b + kNhBlock - 1) / kNhBlock;
-    const int64_t blk0 = nh_first_block(b, t);
-    T qlo[D], qhi[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      qlo[d] = wave_min_f(in ? xi[d] : std::numeric_limits<T>::infinity());
-      qhi[d] = wave_max_f(in ? xi[d] : -std::numeric_limits<T>::infinity());
-    }
-    const uint64_t kmin = wave_min_key_bound(in ? key_i : ~0ull);
-    for (int c0 = 0; c0 < nblk; c0 += kWave) {
-      const T bmax = wave_max_f(in ? bound : -std::numeric_limits<T>::infinity());
-      const int l = lane_id();
-      const bool valid = c0 + l < nblk;
-      int rl = c0 + (valid ? l : 0) + sb;
-      rl -= rl >= nblk ? nblk : 0;
-      const int64_t qb = blk0 + rl;
-      T bl[D], bh[D];
-#pragma unroll
+    // in: the lane's search box holds tile t. The boxes and keys of up to 64 blocks
+    // arrive in one vector load (one block per lane); a wave-wide test against the
+    // box, smallest key and largest bound of the lanes that hold the tile drops
+    // most blocks at once, and each block left is tested per lane before its scan.
+    // The tile's blocks are visited from block sb on, wrapping around: in the own
+    // tile sb is the block that holds the wave's first point, whose neighbours
+    // settle most bounds before the rest of the tile is tested. The result does
+    // not depend on the order (the acceptance test is a strict total order).
+    auto scan_tile = [&](int t, bool in, int sb) {
+      const int b = toff[t];
+      const int nblk = (toff[t + 1] - b + kNhBlock - 1) / kNhBlock;
+      const int64_t blk0 = nh_first_block(b, t);
+      T qlo[D], qhi[D];
+  #pragma unroll
       for (int d = 0; d < D; ++d) {
-        bl[d] = hbox[qb * 2 * D + d];
-        bh[d] = hbox[qb * 2 * D + D + d];
+        qlo[d] = wave_min_f(in ? xi[d] : std::numeric_limits<T>::infinity());
+        qhi[d] = wave_max_f(in ? xi[d] : -std::numeric_limits<T>::infinity());
       }
-      const uint64_t bk = hbkey[qb];
-      uint64_t cand = ballot(valid & block_may_be_higher(bk, kmin, T(0)) &
-                             (metric_lb<D, M>(qlo, qhi, bl, bh, rp) <= bmax));
-      while (cand) {
-        const int i = __builtin_ctzll(cand);
-        cand &= cand - 1ull;
-        T sl[D], sh[D];
-#pragma unroll
+      const uint64_t kmin = wave_min_key_bound(in ? key_i : ~0ull);
+      for (int c0 = 0; c0 < nblk; c0 += kWave) {
+        const T bmax = wave_max_f(in ? bound : -std::numeric_limits<T>::infinity());
+        const int l = lane_id();
+        const bool valid = c0 + l < nblk;
+        int rl = c0 + (valid ? l : 0) + sb;
+        rl -= rl >= nblk ? nblk : 0;
+        const int64_t qb = blk0 + rl;
+        T bl[D], bh[D];
+  #pragma unroll
         for (int d = 0; d < D; ++d) {
-          sl[d] = read_lane(bl[d], i);
-          sh[d] = read_lane(bh[d], i);
+          bl[d] = hbox[qb * 2 * D + d];
+          bh[d] = hbox[qb * 2 * D + D + d];
         }
-        const bool need = in & block_may_be_higher(read_lane(bk, i), key_i, T(0)) &
-                          (metric_lb<D, M>(xi, xi, sl, sh, rp) <= bound);
-        int ri = c0 + i + sb;
-        ri -= ri >= nblk ? nblk : 0;
-        if (ballot(need) != 0ull) scan_block((blk0 + ri) * kNhBlock, in);
+        const uint64_t bk = hbkey[qb];
+        uint64_t cand = ballot(valid & block_may_be_higher(bk, kmin, T(0)) &
+                               (metric_lb<D, M>(qlo, qhi, bl, bh, rp) <= bmax));
+        while (cand) {
+          const int i = __builtin_ctzll(cand);
+          cand &= cand - 1ull;
+          T sl[D], sh[D];
+  #pragma unroll
+          for (int d = 0; d < D; ++d) {
+            sl[d] = read_lane(bl[d], i);
+            sh[d] = read_lane(bh[d], i);
+          }
+          const bool need = in & block_may_be_higher(read_lane(bk, i), key_i, T(0)) &
+                            (metric_lb<D, M>(xi, xi, sl, sh, rp) <= bound);
+          int ri = c0 + i + sb;
+          ri -= ri >= nblk ? nblk : 0;
+          if (ballot(need) != 0ull) scan_block((blk0 + ri) * kNhBlock, in);
+        }
       }
+    };
+
+    // The lanes' own tile first: the nearest candidates tighten the bounds early.
+    // A lane scans it only if its box holds it: on a wrapped dimension a box wider
+    // than half the data's extent can leave it out (the walk below reaches the
+    // own tile at bin b or, wrapped, at b + nper).
+    const int own = uniform(it.tile);  // wave-uniform: keeps the scan's loads scalar
+    {
+      bool in_own = active;
+      int r = own - ev.t_base;
+  #pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        const int b = r % ev.nper;
+        r /= ev.nper;
+        in_own = in_own && (((lo[d] <= b) && (b <= hi[d])) ||
+                            (rp.wrapped[d] && (lo[d] <= b + ev.nper) && (b + ev.nper <= hi[d])));
+      }
+      if (ballot(in_own) != 0ull) scan_tile(own, in_own, uniform(it.begin - toff[own]) / kNhBlock);
+    }
+    walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
+      if (t != own) scan_tile(t, in, 0);
+    });
+    if (active) {
+      snh[s] = nh_slot >= 0 ? hidx[nh_slot] : -1;
+      sdelta[s] = delta;
     }
   };
-
-  // The lanes' own tile first: the nearest candidates tighten the bounds early.
-  // A lane scans it only if its box holds it: on a wrapped dimension a box wider
-  // than half the data's extent can leave it out (the walk below reaches the
-  // own tile at bin b or, wrapped, at b + nper).
-  const int own = uniform(it.tile);  // wave-uniform: keeps the scan's loads scalar
-  {
-    bool in_own = active;
-    int r = own - ev.t_base;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      const int b = r % ev.nper;
-      r /= ev.nper;
-      in_own = in_own && (((lo[d] <= b) && (b <= hi[d])) ||
-                          (rp.wrapped[d] && (lo[d] <= b + ev.nper) && (b + ev.nper <= hi[d])));
-    }
-    if (ballot(in_own) != 0ull) scan_tile(own, in_own, uniform(it.begin - toff[own]) / kNhBlock);
-  }
-  walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
-    if (t != own) scan_tile(t, in, 0);
-  });
-  if (active) {
-    snh[s] = nh_slot >= 0 ? hidx[nh_slot] : -1;
-    sdelta[s] = delta;
+  if constexpr (Persistent) {
+    ItemCursor cursor(n_items, kQueueNh, rp);
+    for (int item = cursor.pull(); item >= 0; item = cursor.pull()) process(item);
+  } else {
+    const int item = xcd_block(blockIdx.x, gridDim.x, rp.xcd_chunk_log2) * (blockDim.x / kWave) +
+                     threadIdx.x / kWave;
+    if (item < *n_items) process(item);
   }
 }
 
@@ -1814,6 +1864,21 @@ static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restric
 // ---------------------------------------------------------------------------
 // Dispatch over (D, metric, kernel).
 // ---------------------------------------------------------------------------
+// Compute units of the current device (cached per device).
+inline int device_cus() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cus[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cus[dev] = v;
+  }
+  return cus[dev];
+}
+
 template <class T>
 struct NeighbourLaunch {
   const RunParamsT<T>* rp;
@@ -1824,12 +1889,26 @@ struct NeighbourLaunch {
   hipStream_t s;
 };
 
+// Persistent waves (the device work-item queues) are built for 1-3 dimensions,
+// the ones the tuning table covers; other dimensions take the grid form.
+template <int D>
+constexpr bool kPersistentBuilt = D <= 3;
+
 template <int D, int M, int K, class T>
 static void launch_density_dmk(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
-  hipLaunchKernelGGL((k_density<D, M, K, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
-                     b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w,
-                     b.sorted.fw, b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
+                       b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w, b.sorted.fw,
+                       b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
+  };
+  if constexpr (kPersistentBuilt<D>) {
+    if (L.rp->persistent > 0) {
+      go(k_density<D, M, K, true, T>);
+      return;
+    }
+  }
+  go(k_density<D, M, K, false, T>);
 }
 
 template <int D, int M, class T>
@@ -1878,10 +1957,19 @@ static void launch_nh_block_keys(const NeighbourLaunch<T>& L) {
 template <int D, int M, class T>
 static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
-  hipLaunchKernelGGL((k_nh_blocks<D, M, T>), dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events,
-                     b.geo, b.toff, b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0],
-                     b.sorted.u, b.sorted.tag, b.srho, b.skey, b.qfine, b.hx, b.h_stride,
-                     b.hkey, b.hidx, b.hbox, b.hbkey, b.snh, b.sdelta);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
+                       b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0], b.sorted.u,
+                       b.sorted.tag, b.srho, b.skey, b.qfine, b.hx, b.h_stride, b.hkey, b.hidx,
+                       b.hbox, b.hbkey, b.snh, b.sdelta);
+  };
+  if constexpr (kPersistentBuilt<D>) {
+    if (L.rp->persistent > 0) {
+      go(k_nh_blocks<D, M, true, T>);
+      return;
+    }
+  }
+  go(k_nh_blocks<D, M, false, T>);
 }
 
 // Nearest-higher over the block layout: the layout pass, then the scan. Its
@@ -2033,8 +2121,12 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
                      s, bb.toff, bb.n_tiles_total, bb.icount);
   scan_exclusive(bb.icount, bb.icount, bb.n_tiles_total, bb.scan_tmp, bb.n_items, s);
-  hipLaunchKernelGGL(k_item_fill, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items);
+  hipLaunchKernelGGL(k_item_fill,
+                     dim3(static_cast<unsigned>(
+                         (std::max<int64_t>(bb.n_tiles_total, kQueueWords) + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0,
+                     s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items,
+                     bb.n_items);
 
   NeighbourLaunch<T> L;
   L.rp = &rp;
@@ -2042,7 +2134,9 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   L.cs = bb.sorted_stride;
   const int waves = rp.block_waves;
   L.block = static_cast<unsigned>(waves * kWave);
-  const int64_t blocks = (bb.max_items + waves - 1) / waves;
+  int64_t blocks = (bb.max_items + waves - 1) / waves;
+  if (rp.persistent > 0 && D <= 3)  // persistent waves (kPersistentBuilt): rp.persistent per CU
+    blocks = std::min<int64_t>(blocks, (static_cast<int64_t>(device_cus()) * rp.persistent + waves - 1) / waves);
   L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
   if (bb.qperm) launch_query_order(L);  // query orders of the two neighbour stages

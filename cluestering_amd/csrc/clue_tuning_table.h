@@ -12,18 +12,19 @@ struct TuningEntry {
   int32_t group_queries;
   int32_t xcd_chunk_log2;
   int32_t block_waves;
+  int32_t persistent;  // persistent waves per CU (0: hardware dispatch)
 };
 
 inline constexpr TuningEntry kTuningTable[] = {
-    {2, 2048, 0, 8, 1},
-    {2, 16384, 1, 4, 1},
-    {2, 131072, 1, 6, 1},
-    {2, INT64_MAX, 1, 6, 1},
-    {3, 2048, 1, 6, 1},
-    {3, 16384, 1, 6, 1},
-    {3, 131072, 1, 8, 1},
-    {3, INT64_MAX, 1, 6, 1},
-    {0, INT64_MAX, 1, 4, 4},  // other ndim
+    {2, 2048, 0, 8, 1, 0},
+    {2, 16384, 1, 4, 1, 0},
+    {2, 131072, 1, 6, 1, 0},
+    {2, INT64_MAX, 1, 6, 1, 0},
+    {3, 2048, 1, 6, 1, 0},
+    {3, 16384, 1, 6, 1, 0},
+    {3, 131072, 1, 8, 1, 0},
+    {3, INT64_MAX, 1, 6, 1, 0},
+    {0, INT64_MAX, 1, 4, 4, 0},  // other ndim
 };
 
 }  // namespace clue_rt

@@ -111,6 +111,7 @@ struct RunParamsT {
   int32_t xcd_chunk_log2;  // neighbour stages: 2^k consecutive blocks per XCD turn; < 0: none
   int32_t block_waves;     // neighbour stages: waves per workgroup (1, 2, 4, 8 or 16)
   int32_t nh_blocks;       // nearest-higher over the Morton-ordered block layout
+  int32_t persistent;      // neighbour stages: persistent waves per CU pulling items (0: none)
 };
 using RunParams = RunParamsT<float>;
 

@@ -68,11 +68,13 @@ class Context:
     def set_tuning(self, group_queries: Optional[bool] = None,
                    xcd_chunk_log2: Optional[int] = None,
                    block_waves: Optional[int] = None,
-                   nh_plain_scan: Optional[bool] = None) -> None:
+                   nh_plain_scan: Optional[bool] = None,
+                   persistent: Optional[int] = None) -> None:
         """Fix the launch tuning of later runs (speed only; results never change).
-        With no arguments the built-in table chooses again."""
+        persistent: waves per CU that pull work items from device queues (0: one
+        wave per item). With no arguments the built-in table chooses again."""
         if (group_queries is None and xcd_chunk_log2 is None and block_waves is None
-                and nh_plain_scan is None):
+                and nh_plain_scan is None and persistent is None):
             _capi.check(self._lib.clue_set_tuning(self.handle, None))
             return
         t = _capi.Tuning()
@@ -80,6 +82,7 @@ class Context:
         t.xcd_chunk_log2 = 4 if xcd_chunk_log2 is None else int(xcd_chunk_log2)
         t.block_waves = 4 if block_waves is None else int(block_waves)
         t.nh_plain_scan = 1 if nh_plain_scan else 0
+        t.persistent = 0 if persistent is None else int(persistent)
         _capi.check(self._lib.clue_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self, params: _capi.Params, sizes: Sequence[int]) -> dict:
@@ -90,7 +93,8 @@ class Context:
                                               sz.ctypes.data_as(C.POINTER(C.c_uint32)),
                                               len(sz), C.byref(t)))
         return {"group_queries": bool(t.group_queries), "xcd_chunk_log2": int(t.xcd_chunk_log2),
-                "block_waves": int(t.block_waves), "nh_plain_scan": bool(t.nh_plain_scan)}
+                "block_waves": int(t.block_waves), "nh_plain_scan": bool(t.nh_plain_scan),
+                "persistent": int(t.persistent)}
 
     def set_profiling(self, enabled: bool) -> None:
         _capi.check(self._lib.clue_set_profiling(self.handle, 1 if enabled else 0))

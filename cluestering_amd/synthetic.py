@@ -143,6 +143,20 @@ def batch2d_4096(seed: int = 0xC1E0 + 4, n_events: int = 4096, lo: int = 256,
     return Workload("batch2d_4096", c, w, 1.5, 5.0, 2.0, GaussianKernel(1.0, 1.0), sizes=sizes)
 
 
+def blobs_nd(rng, n, ndim, n_blobs=None, box=100.0, sigma=1.0, noise=0.2):
+    """Gaussian blobs in ndim dimensions (centres uniform in [0, box]^ndim, 1-sigma
+    spread) plus a uniform noise share, weights uniform in [0.5, 1.5]."""
+    if n_blobs is None:
+        n_blobs = int(rng.integers(5, 51))
+    per = int(np.floor((1.0 - noise) * n / n_blobs))
+    centres = rng.uniform(0.0, box, size=(n_blobs, ndim))
+    parts = [rng.normal(centres[b], sigma, size=(per, ndim)) for b in range(n_blobs)]
+    rest = rng.uniform(0.0, box, size=(n - per * n_blobs, ndim))
+    c = np.concatenate(parts + [rest]).T.astype(np.float32)
+    w = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
+    return np.ascontiguousarray(c), w
+
+
 def served_event(rng, lo=256, hi=262144):
     """One event of the served stream: ndim 2 or 3 with equal probability, size
     log-uniform in [lo, hi]; 2-D events as in batch2d_4096, 3-D as detector3d_1m."""

@@ -253,7 +253,10 @@ typedef struct {
   int32_t nh_plain_scan;  /* 1: nearest-higher scans whole tiles in sorted order; 0 (default): over
                              Morton-ordered candidate blocks, skipping blocks that cannot hold the
                              answer, whenever tags are the default ones and the metric allows it */
-  int32_t reserved[4];
+  int32_t persistent;     /* 0: one wave per work item, dispatched by the hardware; k > 0: k
+                             persistent waves per CU pull (event, tile) work items from per-XCD
+                             device queues (density and the block nearest-higher scan) */
+  int32_t reserved[3];
 } clue_tuning;
 /* Fixes the tuning of the context's later runs and plans; NULL returns to the table. */
 clue_status clue_set_tuning(clue_ctx* ctx, const clue_tuning* tuning);

@@ -90,15 +90,19 @@ def test_batched_gaussian(gpu_ctx, oracle_mod):
     assert gpu["n_clusters"] == ref.n_seeds
 
 
-@pytest.mark.parametrize("tuning", [(0, -1, 4), (1, -1, 4), (1, 0, 4), (1, 4, 4), (0, 6, 4),
-                                    (1, 12, 4), (1, 2, 8), (1, 4, 16), (0, -1, 16)])
+@pytest.mark.parametrize("tuning", [(0, -1, 4, 0), (1, -1, 4, 0), (1, 0, 4, 0), (1, 4, 4, 0),
+                                    (0, 6, 4, 0), (1, 12, 4, 0), (1, 2, 8, 0), (1, 4, 16, 0),
+                                    (0, -1, 16, 0), (1, 4, 1, 1), (1, 6, 1, 32), (1, -1, 2, 8),
+                                    (0, 12, 4, 16)])
 def test_tunings_do_not_change_results(gpu_ctx, oracle_mod, tuning):
     """Every launch tuning (query grouping on/off, XCD chunking, waves per
-    workgroup) gives the same bits: a 2-D gaussian batch with dense tiles and a
-    3-D periodic exponential event."""
+    workgroup, persistent waves pulling from the work-item queues) gives the same
+    bits: a 2-D gaussian batch with dense tiles and a 3-D periodic exponential
+    event."""
     from cluestering_amd import _capi
-    group, xcd, waves = tuning
-    gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd, block_waves=waves)
+    group, xcd, waves, persistent = tuning
+    gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd, block_waves=waves,
+                       persistent=persistent)
     try:
         for w, name in [(synthetic.batch2d_4096(n_events=12, lo=4096, hi=30000), "batch12"),
                         (synthetic.detector3d_1m(n=1 << 15), "detector3d_32k")]:
