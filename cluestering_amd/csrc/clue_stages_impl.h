@@ -733,25 +733,22 @@ __device__ __forceinline__ void wave_memory_sync() {
 
 // Stable counting sort of the points at positions [b, e) of `order` (identity
 // when NULL) by a Bits-bit key, into out[b, e). One wave; cnt is its LDS
-// counter array of 2^Bits entries. Equal keys keep their input order: rounds and
-// lanes are visited in position order and ranked by ballot.
+// counter array of 2^Bits entries. The counts need no order (one LDS atomic per
+// point); the placement visits rounds and lanes in position order and ranks equal
+// keys by ballot, so equal keys keep their input order. Placement takes four
+// rounds' positions and keys with their loads in flight together, then ranks
+// the rounds one after the other.
 template <int Bits, class KeyOf>
 __device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* order,
                                                    int32_t* cnt, KeyOf key_of, int32_t* out) {
   constexpr int kKeys = 1 << Bits;
+  constexpr int kRounds = 4;
   const int lane = lane_id();
   for (int c = lane; c < kKeys; c += kWave) cnt[c] = 0;
   __builtin_amdgcn_wave_barrier();
-  for (int i0 = b; i0 < e; i0 += kWave) {
-    const int i = i0 + lane;
-    const bool valid = i < e;
-    const int s = valid ? (order ? order[i] : i) : 0;
-    const uint32_t k = valid ? key_of(s) : 0u;
-    const uint64_t peers = match_peers(k, Bits, valid);
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lanemask_lt()) == 0) cnt[k] += __popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-  }
+  for (int i = b + lane; i < e; i += kWave) atomicAdd(&cnt[key_of(order ? order[i] : i)], 1);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   int carry = b;
   for (int c0 = 0; c0 < kKeys; c0 += kWave) {
     const int c = c0 + lane;
@@ -766,18 +763,29 @@ __device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* 
     carry += __shfl(x, kWave - 1, 64);
   }
   __builtin_amdgcn_wave_barrier();
-  for (int i0 = b; i0 < e; i0 += kWave) {
-    const int i = i0 + lane;
-    const bool valid = i < e;
-    const int s = valid ? (order ? order[i] : i) : 0;
-    const uint32_t k = valid ? key_of(s) : 0u;
-    const uint64_t peers = match_peers(k, Bits, valid);
-    int pos = 0;
-    if (valid) pos = cnt[k] + __popcll(peers & lanemask_lt());
-    __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lanemask_lt()) == 0) cnt[k] += __popcll(peers);
-    __builtin_amdgcn_wave_barrier();
-    if (valid) out[pos] = s;
+  for (int i0 = b; i0 < e; i0 += kRounds * kWave) {
+    int sr[kRounds];
+    uint32_t kr[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      const int i = i0 + r * kWave + lane;
+      sr[r] = i < e ? (order ? order[i] : i) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) kr[r] = (i0 + r * kWave + lane < e) ? key_of(sr[r]) : 0u;
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+      if (i0 + r * kWave >= e) break;  // wave-uniform
+      const bool valid = i0 + r * kWave + lane < e;
+      const uint32_t k = kr[r];
+      const uint64_t peers = match_peers(k, Bits, valid);
+      int pos = 0;
+      if (valid) pos = cnt[k] + __popcll(peers & lanemask_lt());
+      __builtin_amdgcn_wave_barrier();
+      if (valid && (peers & lanemask_lt()) == 0) cnt[k] += __popcll(peers);
+      __builtin_amdgcn_wave_barrier();
+      if (valid) out[pos] = sr[r];
+    }
   }
 }
 
@@ -785,6 +793,52 @@ __device__ __forceinline__ float fmin_t(float a, float b) { return __builtin_fmi
 __device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_fmin(a, b); }
 __device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
+
+// Minimum and maximum over each row of 16 lanes, in every lane of the row (float
+// through DPP lane moves: pair swap, quad swap, half-row mirror, row mirror; lanes
+// a move does not write take the identity, so each move fuses into its operation).
+template <class F>
+__device__ __forceinline__ float row_reduce_dpp(float v, float id, F op) {
+  const int i = __float_as_int(id);
+  int x = __float_as_int(v);
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0xB1, 0xF, 0xF, false))));
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x4E, 0xF, 0xF, false))));
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x141, 0xF, 0xF, false))));
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x140, 0xF, 0xF, false))));
+  return __int_as_float(x);
+}
+__device__ __forceinline__ float row_min(float v) {
+  return row_reduce_dpp(v, __builtin_inff(), [](float a, float b) { return __builtin_fminf(a, b); });
+}
+__device__ __forceinline__ float row_max(float v) {
+  return row_reduce_dpp(v, -__builtin_inff(), [](float a, float b) { return __builtin_fmaxf(a, b); });
+}
+__device__ __forceinline__ double row_min(double v) {
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) v = fmin_t(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double row_max(double v) {
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) v = fmax_t(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// Largest 64-bit key over each row of 16 lanes (both halves moved by DPP).
+template <int Ctrl>
+__device__ __forceinline__ uint64_t max_dpp_u64(uint64_t v) {
+  const int lo = static_cast<int>(static_cast<uint32_t>(v));
+  const int hi = static_cast<int>(static_cast<uint32_t>(v >> 32));
+  const uint32_t olo = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, lo, Ctrl, 0xF, 0xF, false));
+  const uint32_t ohi = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, hi, Ctrl, 0xF, 0xF, false));
+  const uint64_t o = (static_cast<uint64_t>(ohi) << 32) | olo;
+  return o > v ? o : v;
+}
+__device__ __forceinline__ uint64_t row_max_u64(uint64_t v) {
+  v = max_dpp_u64<0xB1>(v);   // quad_perm 1,0,3,2
+  v = max_dpp_u64<0x4E>(v);   // quad_perm 2,3,0,1
+  v = max_dpp_u64<0x141>(v);  // row_half_mirror
+  return max_dpp_u64<0x140>(v);  // row_mirror
+}
 
 // First layout block of tile t whose candidates begin at sorted position b:
 // tile starts rounded up to whole blocks, one spare block per tile, so that the
@@ -804,6 +858,7 @@ __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int
                                                 T* __restrict__ hx, int64_t hs,
                                                 int32_t* __restrict__ hidx, T* __restrict__ hbox) {
   static_assert(kWave % kNhBlock == 0, "blocks must not straddle waves");
+  static_assert(kNhBlock == 16, "block boxes are reduced over DPP rows of 16 lanes");
   const int lane = lane_id();
   const int64_t blk0 = nh_first_block(b, t);
   const int span = (cnt + kNhBlock - 1) / kNhBlock * kNhBlock;
@@ -828,12 +883,9 @@ __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int
       hi[d] = real ? x[d] : -std::numeric_limits<T>::infinity();
     }
 #pragma unroll
-    for (int off = 1; off < kNhBlock; off <<= 1) {
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        lo[d] = fmin_t(lo[d], __shfl_xor(lo[d], off, 64));
-        hi[d] = fmax_t(hi[d], __shfl_xor(hi[d], off, 64));
-      }
+    for (int d = 0; d < D; ++d) {
+      lo[d] = row_min(lo[d]);
+      hi[d] = row_max(hi[d]);
     }
     if (k < span && (k % kNhBlock) == 0) {
       const int64_t q = blk0 + k / kNhBlock;
@@ -897,6 +949,7 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       r /= ev.nper;
     }
   }
+#pragma unroll 4
   for (int s = b + lane; s < e; s += kWave) {
     T xi[D], si[D];
 #pragma unroll
@@ -1317,10 +1370,6 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
 // monotone), so the results are those of the plain scan.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
-  return static_cast<uint64_t>(__shfl_xor(static_cast<long long>(v), m, 64));
-}
-
 // Keys of the nearest-higher layout (the ordering key of each slot's point, 0 for
 // padding: never higher) and the largest key of every block, once the density
 // stage has written the keys. One thread per slot; a block's 16 slots are 16
@@ -1339,12 +1388,8 @@ static __global__ __launch_bounds__(kBlock) void k_nh_block_keys(const int32_t* 
     k = (s >= 0 && s < n_points) ? skey[s] : 0ull;
     hkey[p] = k;
   }
-  uint64_t m = k;
-#pragma unroll
-  for (int off = 1; off < kNhBlock; off <<= 1) {
-    const uint64_t o = shfl_xor_u64(m, off);
-    m = o > m ? o : m;
-  }
+  static_assert(kNhBlock == 16, "block maxima are reduced over DPP rows of 16 lanes");
+  const uint64_t m = row_max_u64(k);
   if (p < n_slots && (p % kNhBlock) == 0) hbkey[p / kNhBlock] = m;
 }
 
@@ -1884,7 +1929,8 @@ struct NeighbourLaunch {
   const RunParamsT<T>* rp;
   const BatchBuffersT<T>* bb;
   int64_t cs;
-  unsigned grid;
+  unsigned grid;             // one wave per work item (the grid form)
+  unsigned grid_persistent;  // rp.persistent waves per CU (kernels with persistent variants)
   unsigned block;  // threads per workgroup of the neighbour stages
   hipStream_t s;
 };
@@ -1897,18 +1943,18 @@ constexpr bool kPersistentBuilt = D <= 3;
 template <int D, int M, int K, class T>
 static void launch_density_dmk(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
+  auto go = [&](auto kern, unsigned grid) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
                        b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w, b.sorted.fw,
                        b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
   };
   if constexpr (kPersistentBuilt<D>) {
     if (L.rp->persistent > 0) {
-      go(k_density<D, M, K, true, T>);
+      go(k_density<D, M, K, true, T>, L.grid_persistent);
       return;
     }
   }
-  go(k_density<D, M, K, false, T>);
+  go(k_density<D, M, K, false, T>, L.grid);
 }
 
 template <int D, int M, class T>
@@ -1957,19 +2003,19 @@ static void launch_nh_block_keys(const NeighbourLaunch<T>& L) {
 template <int D, int M, class T>
 static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
-  auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(L.grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
+  auto go = [&](auto kern, unsigned grid) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
                        b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0], b.sorted.u,
                        b.sorted.tag, b.srho, b.skey, b.qfine, b.hx, b.h_stride, b.hkey, b.hidx,
                        b.hbox, b.hbkey, b.snh, b.sdelta);
   };
   if constexpr (kPersistentBuilt<D>) {
     if (L.rp->persistent > 0) {
-      go(k_nh_blocks<D, M, true, T>);
+      go(k_nh_blocks<D, M, true, T>, L.grid_persistent);
       return;
     }
   }
-  go(k_nh_blocks<D, M, false, T>);
+  go(k_nh_blocks<D, M, false, T>, L.grid);
 }
 
 // Nearest-higher over the block layout: the layout pass, then the scan. Its
@@ -2134,10 +2180,12 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   L.cs = bb.sorted_stride;
   const int waves = rp.block_waves;
   L.block = static_cast<unsigned>(waves * kWave);
-  int64_t blocks = (bb.max_items + waves - 1) / waves;
-  if (rp.persistent > 0 && D <= 3)  // persistent waves (kPersistentBuilt): rp.persistent per CU
-    blocks = std::min<int64_t>(blocks, (static_cast<int64_t>(device_cus()) * rp.persistent + waves - 1) / waves);
+  const int64_t blocks = (bb.max_items + waves - 1) / waves;
   L.grid = static_cast<unsigned>((blocks + kXcds - 1) / kXcds * kXcds);
+  // Persistent waves: rp.persistent per CU, never more than the grid form's.
+  const int64_t pblocks = std::min<int64_t>(
+      blocks, (static_cast<int64_t>(device_cus()) * std::max(rp.persistent, 0) + waves - 1) / waves);
+  L.grid_persistent = static_cast<unsigned>((pblocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
   if (bb.qperm) launch_query_order(L);  // query orders of the two neighbour stages
 

@@ -93,12 +93,14 @@ def test_batched_gaussian(gpu_ctx, oracle_mod):
 @pytest.mark.parametrize("tuning", [(0, -1, 4, 0), (1, -1, 4, 0), (1, 0, 4, 0), (1, 4, 4, 0),
                                     (0, 6, 4, 0), (1, 12, 4, 0), (1, 2, 8, 0), (1, 4, 16, 0),
                                     (0, -1, 16, 0), (1, 4, 1, 1), (1, 6, 1, 32), (1, -1, 2, 8),
-                                    (0, 12, 4, 16)])
+                                    (0, 12, 4, 16), (0, -1, 1, 1)])
 def test_tunings_do_not_change_results(gpu_ctx, oracle_mod, tuning):
     """Every launch tuning (query grouping on/off, XCD chunking, waves per
     workgroup, persistent waves pulling from the work-item queues) gives the same
     bits: a 2-D gaussian batch with dense tiles and a 3-D periodic exponential
-    event."""
+    event. (0, -1, 1, 1): one persistent wave per CU, far fewer than the batch's
+    work items, beside the nearest-higher plain scan, which has no persistent
+    form and keeps one wave per item."""
     from cluestering_amd import _capi
     group, xcd, waves, persistent = tuning
     gpu_ctx.set_tuning(group_queries=bool(group), xcd_chunk_log2=xcd, block_waves=waves,
