@@ -51,7 +51,7 @@ enum Slot {
   S_EVENTS, S_CHUNKS, S_EXT, S_GEO, S_K0, S_V0, S_K1, S_V1, S_HIST, S_SCAN, S_TOFF, S_ICOUNT,
   S_NITEMS, S_ITEMS, S_SRHO, S_SKEY, S_SNH, S_SDELTA, S_FLAG, S_RANK, S_NSEEDS, S_SEEDS, S_SX,
   S_SW, S_SSIG, S_SU, S_STAG, S_SFW, S_PERM, S_RHO, S_NH, S_ISSEED, S_DELTA, S_HIN, S_HCL,
-  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_ESEEDS, S_EOFF, S_IPERM, S_NUM
+  S_QPERM, S_QPERM_NH, S_QFINE, S_QKEY, S_HX, S_HKEY, S_HIDX, S_HBOX, S_HBKEY, S_EORDER, S_ESEEDS, S_EOFF, S_IPERM, S_SBOX, S_NUM
 };
 
 // Host-side description of one batch (all derived from the event sizes).
@@ -178,6 +178,7 @@ struct WorkspaceShape {
   bool own_rho = false;            // results the caller does not provide live in the workspace
   bool own_nh = false;
   bool own_is_seed = false;
+  bool sorted_boxes = false;       // density culls candidate blocks by their bounding boxes
 };
 
 // Bytes of every workspace slot for a batch of this shape (0: unused).
@@ -216,6 +217,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_HIDX] = sizeof(int32_t) * h_slots;
   bytes[S_HBOX] = T * h_blocks * 2 * D;
   bytes[S_HBKEY] = sizeof(uint64_t) * h_blocks;
+  bytes[S_SBOX] = w.sorted_boxes ? T * ((n + clue_dev::kSortedBox - 1) / clue_dev::kSortedBox) * 2 * D : 0;
   bytes[S_SRHO] = T * n;
   bytes[S_SKEY] = sizeof(uint64_t) * n;
   bytes[S_SFW] = w.flat_kernel ? T * n : 0;

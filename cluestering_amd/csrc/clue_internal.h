@@ -105,6 +105,9 @@ struct BatchBuffersT {
   T* hbox;           // [h_blocks][2][D] bounding box (lower corner, upper corner)
   uint64_t* hbkey;   // [h_blocks] largest key of the block
   int64_t h_stride;  // slots = h_blocks * kNhBlock
+  // Bounding box of each kSortedBox consecutive tile-sorted positions (NULL when
+  // the metric has no box bound): [ceil(n / kSortedBox)][2][D].
+  T* sbox;
 };
 using BatchBuffers = BatchBuffersT<float>;
 

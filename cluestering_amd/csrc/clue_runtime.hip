@@ -83,6 +83,18 @@ clue_status validate(const P* p) {
 // it needs the Morton order of the query grouping, candidates visited in any
 // order (unique tags: the default ones), and a metric with a lower bound over a
 // bounding box (not Mahalanobis; weighted metrics with weights >= 0).
+// Bounding boxes of each kSortedBox tile-sorted positions, for the density stage's
+// block culling: every metric with a lower bound over a box (not Mahalanobis),
+// weighted metrics with weights >= 0.
+template <class P>
+bool sorted_boxes_enabled(const P* p) {
+  if (p->metric == CLUE_METRIC_MAHALANOBIS) return false;
+  if (p->metric == CLUE_METRIC_WEIGHTED_EUCLIDEAN || p->metric == CLUE_METRIC_WEIGHTED_CHEBYSHEV)
+    for (int d = 0; d < p->ndim; ++d)
+      if (!(p->mparam[d] >= 0)) return false;
+  return true;
+}
+
 template <class P, class Pts>
 bool nh_blocks_enabled(const P* p, const Pts* pts, const BatchPlan& bp) {
   if (bp.tuning.nh_plain_scan || !bp.tuning.group_queries || pts->tags != nullptr) return false;
@@ -198,6 +210,7 @@ WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, 
   w.own_rho = res->rho == nullptr;
   w.own_nh = res->nearest_higher == nullptr;
   w.own_is_seed = res->is_seed == nullptr;
+  w.sorted_boxes = sorted_boxes_enabled(p);
   return w;
 }
 
@@ -277,6 +290,7 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   b.hbox = nhb ? at<T>(slot, S_HBOX) : nullptr;
   b.hbkey = nhb ? at<uint64_t>(slot, S_HBKEY) : nullptr;
   b.h_stride = h_slots;
+  b.sbox = sorted_boxes_enabled(p) ? at<T>(slot, S_SBOX) : nullptr;
   for (int d = 0; d < kMaxDim; ++d) {
     b.sorted.x[d] = d < D ? at<T>(slot, S_SX) + stride * d : nullptr;
     b.sorted.sigma[d] = (has_sigma && d < D) ? at<T>(slot, S_SSIG) + stride * d : nullptr;

@@ -898,6 +898,69 @@ __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int
   }
 }
 
+// Bounding box of each kSortedBox consecutive tile-sorted positions (positions
+// past n hold nothing). A box may cover the end of one tile and the start of the
+// next, or of two events: it then bounds more points than a walk takes from it,
+// which only makes it skipped less often. Float: a thread takes four positions
+// with one 16-byte load per column (the columns start 64-aligned) and four lanes
+// make a box; double: sixteen lanes, one position each.
+template <class F>
+__device__ __forceinline__ float quad_reduce_dpp(float v, float id, F op) {
+  const int i = __float_as_int(id);
+  int x = __float_as_int(v);
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0xB1, 0xF, 0xF, false))));
+  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x4E, 0xF, 0xF, false))));
+  return __int_as_float(x);
+}
+
+template <int D, class T>
+__global__ __launch_bounds__(kBlock) void k_sorted_boxes(const T* __restrict__ sx, int64_t cs,
+                                                         int n, T* __restrict__ sbox) {
+  static_assert(kSortedBox == 16, "a box is four lanes of four floats, or one row of 16 lanes");
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if constexpr (sizeof(T) == 4) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const int64_t p = 4 * t;  // first of the thread's four positions
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float lo = __builtin_inff(), hi = -__builtin_inff();
+      if (p + 3 < n) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(sx + d * cs + p);
+        lo = __builtin_fminf(__builtin_fminf(v.x, v.y), __builtin_fminf(v.z, v.w));
+        hi = __builtin_fmaxf(__builtin_fmaxf(v.x, v.y), __builtin_fmaxf(v.z, v.w));
+      } else {
+        for (int k = 0; k < 4; ++k)
+          if (p + k < n) {
+            const float x = sx[d * cs + p + k];
+            lo = __builtin_fminf(lo, x);
+            hi = __builtin_fmaxf(hi, x);
+          }
+      }
+      lo = quad_reduce_dpp(lo, __builtin_inff(), [](float a, float b) { return __builtin_fminf(a, b); });
+      hi = quad_reduce_dpp(hi, -__builtin_inff(), [](float a, float b) { return __builtin_fmaxf(a, b); });
+      if (p < n && (t % 4) == 0) {
+        const int64_t q = p / kSortedBox;
+        sbox[q * 2 * D + d] = lo;
+        sbox[q * 2 * D + D + d] = hi;
+      }
+    }
+  } else {
+    const int64_t p = t;
+    const bool real = p < n;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T x = real ? sx[d * cs + p] : T(0);
+      const T lo = row_min(real ? x : std::numeric_limits<T>::infinity());
+      const T hi = row_max(real ? x : -std::numeric_limits<T>::infinity());
+      if (real && (p % kSortedBox) == 0) {
+        const int64_t q = p / kSortedBox;
+        sbox[q * 2 * D + d] = lo;
+        sbox[q * 2 * D + D + d] = hi;
+      }
+    }
+  }
+}
+
 // Both query orders of one tile per wave. Pass 1 packs each point's keys into
 // qkey (Morton index in bits 0-7, class for dc in 8-13, class for dm in 14-19);
 // then three stable counting sorts: sorted order -> Morton order (qfine), and
@@ -1043,6 +1106,62 @@ __device__ __forceinline__ void load_block(const T* __restrict__ p, T (&out)[U])
   for (int k = 0; k < U; ++k) out[k] = v[k];
 }
 
+// A lower bound, over every query point in [qlo, qhi] and every candidate inside
+// the box [bl, bh], of the value metric_value computes (the same operations on
+// the per-dimension gaps, which never exceed the pairs' own differences after
+// rounding). With qlo = qhi = x it bounds one query point. Weighted metrics
+// need weights >= 0 (checked on the host).
+template <int D, int M, class T>
+__device__ __forceinline__ T metric_lb(const T (&qlo)[D], const T (&qhi)[D], const T (&bl)[D],
+                                       const T (&bh)[D], const RunParamsT<T>& rp) {
+  T g[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) g[d] = fmax_t(fmax_t(bl[d] - qhi[d], qlo[d] - bh[d]), T(0));
+  if constexpr (M == M_EUCLID) {
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] * g[d] : g[d] * g[d] + s;
+    return s;
+  } else if constexpr (M == M_WEUCLID) {
+    T s = T(0);
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = rp.mparam[d] * g[d] * g[d] + s;
+    return s;
+  } else if constexpr (M == M_PERIODIC) {
+    // a = |dx| lies in [g, G]. With period P > 0, min(a, P - a) >= min(g, P - G)
+    // when P - G >= 0 (else 0 bounds its square). With P <= 0 (an unwrapped
+    // dimension), min(a, P - a) = P - a and |P - a| >= a >= g.
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const T P = rp.mparam[d];
+      const T G = fmax_t(qhi[d] - bl[d], bh[d] - qlo[d]);
+      const T pg = P - G;
+      const T m = (P <= T(0)) ? g[d] : ((pg >= T(0)) ? fmin_t(g[d], pg) : T(0));
+      s = (d == D - 1) ? m * m : m * m + s;
+    }
+    return s;
+  } else if constexpr (M == M_MANHATTAN) {
+    T s;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] : g[d] + s;
+    return s;
+  } else if constexpr (M == M_CHEBYSHEV) {
+    T m = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) m = (m < g[d]) ? g[d] : m;
+    return m;
+  } else {  // M_WCHEBYSHEV
+    T m = T(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T a = rp.mparam[d] * g[d];
+      m = (m < a) ? a : m;
+    }
+    return m;
+  }
+}
+
 // Compile-time loop modes of the density stage: kernel exp arguments known to be
 // in range, and square roots known to need no tiny-input branch.
 template <bool InRange, bool Normal>
@@ -1072,7 +1191,7 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
     int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
     const T* __restrict__ sw, const T* __restrict__ sfw, const T* __restrict__ ssig,
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
-    T* __restrict__ srho, uint64_t* __restrict__ skey) {
+    T* __restrict__ srho, uint64_t* __restrict__ skey, const T* __restrict__ sbox) {
   constexpr int U = Unroll<D>::U;
   auto process = [&](const int item) {
     const Item it = items[item];
@@ -1148,6 +1267,30 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
       for (; j < j1; ++j)
         add(metric_value<D, M>(xi, si, sx, cs, j, ssig, rp), j, wcol[j], thr, own_c, mode_c);
     };
+    // A neighbour tile's candidates go by boxes of kSortedBox sorted positions: a
+    // box that no lane can reach (a lower bound of the metric over it above every
+    // lane's threshold) holds only candidates beyond every lane's radius, whose
+    // terms the reference adds as +0, so it is skipped without changing the sums.
+    // (The own tile is the dense one: there nearly every box is reached.)
+    auto walk_boxes = [&](int j0, int j1, T thr, auto mode_c) {
+      if constexpr (M != M_MAHALANOBIS) {
+        for (int j = j0; j < j1;) {
+          const int qb = j / kSortedBox;
+          const int je = min(j1, (qb + 1) * kSortedBox);
+          T bl[D], bh[D];
+  #pragma unroll
+          for (int d = 0; d < D; ++d) {
+            bl[d] = sbox[static_cast<int64_t>(qb) * 2 * D + d];
+            bh[d] = sbox[static_cast<int64_t>(qb) * 2 * D + D + d];
+          }
+          if (ballot(metric_lb<D, M>(xi, xi, bl, bh, rp) <= thr) != 0ull)
+            walk(j, je, thr, std::false_type{}, mode_c);
+          j = je;
+        }
+      } else {
+        walk(j0, j1, thr, std::false_type{}, mode_c);
+      }
+    };
     auto run = [&](auto mode_c) {
       walk_tiles<D>(lo, hi, ev.nper, rp.wrapped, ev.t_base, [&](int t, bool in) {
         const int j0 = toff[t];
@@ -1155,6 +1298,8 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
         const T thr = in ? thr_in : T(-1);
         if (t == it.tile)
           walk(j0, j1, thr, std::true_type{}, mode_c);
+        else if (sbox)
+          walk_boxes(j0, j1, thr, mode_c);
         else
           walk(j0, j1, thr, std::false_type{}, mode_c);
       });
@@ -1391,62 +1536,6 @@ static __global__ __launch_bounds__(kBlock) void k_nh_block_keys(const int32_t* 
   static_assert(kNhBlock == 16, "block maxima are reduced over DPP rows of 16 lanes");
   const uint64_t m = row_max_u64(k);
   if (p < n_slots && (p % kNhBlock) == 0) hbkey[p / kNhBlock] = m;
-}
-
-// A lower bound, over every query point in [qlo, qhi] and every candidate inside
-// the box [bl, bh], of the value metric_value computes (the same operations on
-// the per-dimension gaps, which never exceed the pairs' own differences after
-// rounding). With qlo = qhi = x it bounds one query point. Weighted metrics
-// need weights >= 0 (checked on the host).
-template <int D, int M, class T>
-__device__ __forceinline__ T metric_lb(const T (&qlo)[D], const T (&qhi)[D], const T (&bl)[D],
-                                       const T (&bh)[D], const RunParamsT<T>& rp) {
-  T g[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) g[d] = fmax_t(fmax_t(bl[d] - qhi[d], qlo[d] - bh[d]), T(0));
-  if constexpr (M == M_EUCLID) {
-    T s;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] * g[d] : g[d] * g[d] + s;
-    return s;
-  } else if constexpr (M == M_WEUCLID) {
-    T s = T(0);
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) s = rp.mparam[d] * g[d] * g[d] + s;
-    return s;
-  } else if constexpr (M == M_PERIODIC) {
-    // a = |dx| lies in [g, G]. With period P > 0, min(a, P - a) >= min(g, P - G)
-    // when P - G >= 0 (else 0 bounds its square). With P <= 0 (an unwrapped
-    // dimension), min(a, P - a) = P - a and |P - a| >= a >= g.
-    T s;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      const T P = rp.mparam[d];
-      const T G = fmax_t(qhi[d] - bl[d], bh[d] - qlo[d]);
-      const T pg = P - G;
-      const T m = (P <= T(0)) ? g[d] : ((pg >= T(0)) ? fmin_t(g[d], pg) : T(0));
-      s = (d == D - 1) ? m * m : m * m + s;
-    }
-    return s;
-  } else if constexpr (M == M_MANHATTAN) {
-    T s;
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) s = (d == D - 1) ? g[d] : g[d] + s;
-    return s;
-  } else if constexpr (M == M_CHEBYSHEV) {
-    T m = T(0);
-#pragma unroll
-    for (int d = 0; d < D; ++d) m = (m < g[d]) ? g[d] : m;
-    return m;
-  } else {  // M_WCHEBYSHEV
-    T m = T(0);
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const T a = rp.mparam[d] * g[d];
-      m = (m < a) ? a : m;
-    }
-    return m;
-  }
 }
 
 // Wave-wide minimum and maximum of floating-point values, and a lower bound of
@@ -1946,7 +2035,7 @@ static void launch_density_dmk(const NeighbourLaunch<T>& L) {
   auto go = [&](auto kern, unsigned grid) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
                        b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w, b.sorted.fw,
-                       b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey);
+                       b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey, b.sbox);
   };
   if constexpr (kPersistentBuilt<D>) {
     if (L.rp->persistent > 0) {
@@ -2090,6 +2179,33 @@ static void launch_query_order_d(const NeighbourLaunch<T>& L) {
                      b.hidx, b.hbox);
 }
 
+template <int D, class T>
+static void launch_sorted_boxes_d(const NeighbourLaunch<T>& L) {
+  const BatchBuffersT<T>& b = *L.bb;
+  // Float: four positions per thread; double: one.
+  const int64_t per = sizeof(T) == 4 ? 4 : 1;
+  const int64_t n16 = (static_cast<int64_t>(b.n_total) + kSortedBox - 1) / kSortedBox * kSortedBox;
+  const int64_t threads = n16 / per;
+  hipLaunchKernelGGL((k_sorted_boxes<D, T>), dim3(static_cast<unsigned>((threads + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, L.s, b.sorted.x[0], L.cs, b.n_total, b.sbox);
+}
+
+template <class T>
+static void launch_sorted_boxes(const NeighbourLaunch<T>& L) {
+  switch (L.rp->D) {
+    case 1: launch_sorted_boxes_d<1>(L); break;
+    case 2: launch_sorted_boxes_d<2>(L); break;
+    case 3: launch_sorted_boxes_d<3>(L); break;
+    case 4: launch_sorted_boxes_d<4>(L); break;
+    case 5: launch_sorted_boxes_d<5>(L); break;
+    case 6: launch_sorted_boxes_d<6>(L); break;
+    case 7: launch_sorted_boxes_d<7>(L); break;
+    case 8: launch_sorted_boxes_d<8>(L); break;
+    case 9: launch_sorted_boxes_d<9>(L); break;
+    default: launch_sorted_boxes_d<10>(L); break;
+  }
+}
+
 // The query orders of both neighbour stages (qperm for dc, qperm_nh for dm).
 template <class T>
 static void launch_query_order(const NeighbourLaunch<T>& L) {
@@ -2187,6 +2303,7 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
       blocks, (static_cast<int64_t>(device_cus()) * std::max(rp.persistent, 0) + waves - 1) / waves);
   L.grid_persistent = static_cast<unsigned>((pblocks + kXcds - 1) / kXcds * kXcds);
   L.s = s;
+  if (bb.sbox) launch_sorted_boxes(L);  // candidate boxes of the density stage
   if (bb.qperm) launch_query_order(L);  // query orders of the two neighbour stages
 
   // (b), (c)

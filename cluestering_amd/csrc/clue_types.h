@@ -10,6 +10,10 @@ constexpr int kMaxDim = 10;
 // one largest key per block).
 constexpr int kNhBlock = 16;
 
+// Candidates per bounding box of the tile-sorted columns (the density stage
+// skips a box that no lane can reach).
+constexpr int kSortedBox = 16;
+
 // Largest tile count of an event that the fused per-event tile build takes (its
 // per-wave tile counters live in LDS); batches with a larger event take the
 // radix-sort path.
