@@ -188,21 +188,28 @@ __device__ __forceinline__ double expd_cw(double x) {
 // Correctly rounded square root. For s >= 2^-96 the hardware estimate r (within
 // 1 ulp) is settled by Tuckerman's rounding test on its neighbours r - ulp and
 // r + ulp, whose fma residuals s - r' * r carry the sign of the exact error
-// there (no denormal intermediates). Zero, tiny, negative and NaN inputs take
-// the IEEE library sqrt.
-// The test alone, for s >= 2^-96 or s == 0 (with s == 0, r - ulp is a NaN, whose
-// residual fails the test, and r + ulp's residual is +0: r = 0 stands).
+// there (no denormal intermediates). Zero, tiny, negative, infinite and NaN
+// inputs take the IEEE library sqrt.
+// The test alone, for finite s >= 2^-96 or s == 0. The answer is r - ulp when
+// s - (r - ulp) r <= 0, r + ulp when s - (r + ulp) r > 0, else r; the second
+// residual is below the first, so the answer is r - ulp plus one for each
+// residual above zero. A residual is above zero exactly when (r' r - s), the fma
+// with the sign moved (an exact zero is +0 in round-to-nearest), has its sign bit
+// set: two shifts and one three-way integer add replace two compares and two
+// selects. r - ulp saturates at +0, so s == 0 gives 0 (both residuals are +0).
 __device__ __forceinline__ float sqrt_rn_normal(float s) {
   const float r = __builtin_amdgcn_sqrtf(s);
-  const float rm = __int_as_float(__float_as_int(r) - 1);
-  const float rp = __int_as_float(__float_as_int(r) + 1);
-  float q = (__builtin_fmaf(-rm, r, s) <= 0.0f) ? rm : r;
-  q = (__builtin_fmaf(-rp, r, s) > 0.0f) ? rp : q;
-  return q;
+  const uint32_t rb = __float_as_uint(r);
+  const uint32_t mb = __builtin_elementwise_sub_sat(rb, 1u);
+  const float rm = __uint_as_float(mb);
+  const float rp = __uint_as_float(rb + 1u);
+  const uint32_t above_m = __float_as_uint(__builtin_fmaf(rm, r, -s)) >> 31;
+  const uint32_t above_p = __float_as_uint(__builtin_fmaf(rp, r, -s)) >> 31;
+  return __uint_as_float(mb + above_m + above_p);
 }
 __device__ __forceinline__ float sqrt_rn(float s) {
   float q = sqrt_rn_normal(s);
-  if (__builtin_expect(!(s >= 0x1p-96f), 0)) q = __builtin_sqrtf(s);
+  if (__builtin_expect(!(s >= 0x1p-96f && s <= 0x1.fffffep+127f), 0)) q = __builtin_sqrtf(s);
   return q;
 }
 

@@ -45,6 +45,14 @@ def test_folded_gaussian_exp_equals_the_oracle():
     _run("test_exp_scaled")
 
 
+def test_sqrt_sign_selection_is_correctly_rounded():
+    """The float square root of the neighbour stages settles the hardware estimate
+    by the sign bits of two fma residuals added as integers: correctly rounded for
+    every estimate within one ulp, on every float of [1, 4), at 0 and on a stride
+    of all finite floats from 2^-96 up."""
+    _run("test_sqrt_select")
+
+
 @pytest.mark.gpu
 def test_cpp_dropin_program():
     _run("test_clusterer", os.path.join(ROOT, "tests", "golden", "reference_data"))
