@@ -179,6 +179,7 @@ struct WorkspaceShape {
   bool own_nh = false;
   bool own_is_seed = false;
   bool sorted_boxes = false;       // density culls candidate blocks by their bounding boxes
+  bool density_morton = false;     // density takes the Morton order (no class order for dc)
 };
 
 // Bytes of every workspace slot for a batch of this shape (0: unused).
@@ -208,7 +209,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_ICOUNT] = sizeof(int32_t) * NT;
   bytes[S_NITEMS] = sizeof(int32_t) * 32;  // item count, then the work-item queue counters
   bytes[S_ITEMS] = sizeof(Item) * w.max_items;
-  bytes[S_QPERM] = w.group_queries ? sizeof(int32_t) * n : 0;
+  bytes[S_QPERM] = (w.group_queries && !w.density_morton) ? sizeof(int32_t) * n : 0;
   bytes[S_QPERM_NH] = (w.group_queries && w.separate_dm_order) ? sizeof(int32_t) * n : 0;
   bytes[S_QFINE] = w.group_queries ? sizeof(int32_t) * n : 0;
   bytes[S_QKEY] = w.group_queries ? sizeof(uint32_t) * n : 0;
@@ -219,7 +220,7 @@ inline void workspace_requests(const WorkspaceShape& w, size_t (&bytes)[S_NUM]) 
   bytes[S_HBKEY] = sizeof(uint64_t) * h_blocks;
   bytes[S_SBOX] = w.sorted_boxes ? T * ((n + clue_dev::kSortedBox - 1) / clue_dev::kSortedBox) * 2 * D : 0;
   bytes[S_SRHO] = T * n;
-  bytes[S_SKEY] = sizeof(uint64_t) * n;
+  bytes[S_SKEY] = w.nh_blocks ? 0 : sizeof(uint64_t) * n;  // the block scan's keys: S_HKEY
   bytes[S_SFW] = w.flat_kernel ? T * n : 0;
   bytes[S_SNH] = sizeof(int32_t) * n;
   bytes[S_SDELTA] = T * n;

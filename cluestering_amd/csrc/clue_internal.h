@@ -79,12 +79,14 @@ struct BatchBuffersT {
   int32_t* icount;     // [n_tiles_total]
   int32_t* n_items;    // [1]
   Item* items;         // [max_items]
-  int32_t* qperm;      // [n_total] density query order within tiles; NULL = sorted order
+  int32_t* qperm;      // [n_total] density query order within tiles (qfine with candidate
+                       // boxes); NULL = sorted order
   int32_t* qperm_nh;   // [n_total] nearest-higher query order (may alias qperm)
   int32_t* qfine;      // [n_total] Morton order within tiles (input of both)
   uint32_t* qkey;      // [n_total] packed query keys (Morton, class for dc, class for dm)
   T* srho;
   uint64_t* skey;      // ordering key of every sorted point: float (rho, tag); double rho
+                       // (the plain nearest-higher scan; NULL with the block layout)
   int32_t* snh;
   T* sdelta;
   int32_t* flag;       // [n_total]

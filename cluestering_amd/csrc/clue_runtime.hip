@@ -105,6 +105,21 @@ bool nh_blocks_enabled(const P* p, const Pts* pts, const BatchPlan& bp) {
   return true;
 }
 
+// Density's queries in Morton order: with candidate boxes the class order buys
+// nothing (measured on MI355X on the 4096-event 2-D batch: 24.9 ms either way), and
+// the block scan's layout then receives the keys from the density stage itself.
+template <class P>
+bool density_morton(const P* p, const BatchPlan& bp) {
+  return bp.tuning.group_queries && sorted_boxes_enabled(p);
+}
+
+// The plain nearest-higher scan's own (class, Morton) order for dm, when the
+// density stage's order is not that order.
+template <class P>
+bool separate_dm_order(const P* p, const BatchPlan& bp, bool nhb) {
+  return bp.tuning.group_queries && !nhb && (p->dm != p->dc || density_morton(p, bp));
+}
+
 template <class P, class Pts>
 RunParamsT<value_t<P>> make_run_params(const P* p, const Pts* pts, const BatchPlan& bp) {
   using T = value_t<P>;
@@ -203,7 +218,7 @@ WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, 
   w.event_tiles = !bp.event_order.empty();
   w.group_queries = bp.tuning.group_queries != 0;
   w.nh_blocks = nh_blocks_enabled(p, pts, bp);
-  w.separate_dm_order = p->dm != p->dc && !w.nh_blocks;
+  w.separate_dm_order = separate_dm_order(p, bp, w.nh_blocks);
   w.flat_kernel = p->kernel == CLUE_KERNEL_FLAT;
   w.has_sigma = pts->sigma[0] != nullptr;
   w.has_u = pts->density_uncertainty != nullptr;
@@ -211,6 +226,7 @@ WorkspaceShape workspace_shape(const BatchPlan& bp, const P* p, const Pts* pts, 
   w.own_nh = res->nearest_higher == nullptr;
   w.own_is_seed = res->is_seed == nullptr;
   w.sorted_boxes = sorted_boxes_enabled(p);
+  w.density_morton = density_morton(p, bp);
   return w;
 }
 
@@ -265,15 +281,15 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   b.icount = at<int32_t>(slot, S_ICOUNT);
   b.n_items = at<int32_t>(slot, S_NITEMS);
   b.items = at<Item>(slot, S_ITEMS);
-  b.qperm = bp.tuning.group_queries ? at<int32_t>(slot, S_QPERM) : nullptr;
-  // The block scan orders its queries by qfine: no separate order for dm.
-  b.qperm_nh = (bp.tuning.group_queries && p->dm != p->dc && !nhb)
-                   ? at<int32_t>(slot, S_QPERM_NH)
-                   : b.qperm;
   b.qfine = bp.tuning.group_queries ? at<int32_t>(slot, S_QFINE) : nullptr;
+  b.qperm = !bp.tuning.group_queries ? nullptr
+            : density_morton(p, bp)  ? b.qfine
+                                     : at<int32_t>(slot, S_QPERM);
+  // The block scan orders its queries by qfine: no separate order for dm.
+  b.qperm_nh = separate_dm_order(p, bp, nhb) ? at<int32_t>(slot, S_QPERM_NH) : b.qperm;
   b.qkey = bp.tuning.group_queries ? at<uint32_t>(slot, S_QKEY) : nullptr;
   b.srho = at<T>(slot, S_SRHO);
-  b.skey = at<uint64_t>(slot, S_SKEY);
+  b.skey = nhb ? nullptr : at<uint64_t>(slot, S_SKEY);
   b.snh = at<int32_t>(slot, S_SNH);
   b.sdelta = at<T>(slot, S_SDELTA);
   b.flag = at<int32_t>(slot, S_FLAG);
@@ -307,6 +323,9 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
   ra->delta = res->delta;
   ra->event_of_cluster = nullptr;
   ra->local_ids = 0;
+  // The density stage fills the block layout's keys only from the Morton order
+  // (the block scan implies it: no Mahalanobis, weights >= 0, query grouping).
+  if (nhb && b.qperm != b.qfine) return CLUE_ERR_RUNTIME;
   return CLUE_OK;
 }
 

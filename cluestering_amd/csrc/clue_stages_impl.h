@@ -850,8 +850,8 @@ __device__ __forceinline__ int64_t nh_first_block(int b, int t) {
 // The nearest-higher layout of tile t (one wave), written by the query-order
 // pass: the tile's candidates in Morton order (order[k], k < cnt) from block
 // nh_first_block on, padded to whole blocks (index -1), with each block's
-// bounding box. The keys, known after the density stage, are filled in by
-// k_nh_block_keys.
+// bounding box. The keys, known after the density stage, are written by the
+// density stage itself (k_density, from the same Morton order).
 template <int D, class T>
 __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int32_t* order,
                                                 const T* __restrict__ sx, int64_t cs,
@@ -961,11 +961,11 @@ __global__ __launch_bounds__(kBlock) void k_sorted_boxes(const T* __restrict__ s
   }
 }
 
-// Both query orders of one tile per wave. Pass 1 packs each point's keys into
+// The query orders of one tile per wave. Pass 1 packs each point's keys into
 // qkey (Morton index in bits 0-7, class for dc in 8-13, class for dm in 14-19);
-// then three stable counting sorts: sorted order -> Morton order (qfine), and
-// Morton order -> (class, Morton) for dc (qperm) and for dm (qperm_nh, skipped
-// when it aliases qperm).
+// then up to three stable counting sorts: sorted order -> Morton order (qfine),
+// and Morton order -> (class, Morton) for dc (qperm) and for dm (qperm_nh). An
+// order that aliases qfine, or qperm_nh aliasing qperm, is not sorted again.
 template <int D, class T>
 __global__ __launch_bounds__(kBlock) void k_query_order(
     const RunParamsT<T> rp, const EventDesc* __restrict__ events, int n_events,
@@ -983,11 +983,13 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
   if (t >= n_tiles) return;
   const int lane = lane_id();
   const int b = toff[t], e = toff[t + 1];
+  const bool class_dc = qperm != qfine;
+  const bool class_dm = qperm_nh != qperm && qperm_nh != qfine;
   if (e - b <= kWave) {  // one item: the query order is irrelevant
     if (b + lane < e) {
       qfine[b + lane] = b + lane;
-      qperm[b + lane] = b + lane;
-      qperm_nh[b + lane] = b + lane;
+      if (class_dc) qperm[b + lane] = b + lane;
+      if (class_dm) qperm_nh[b + lane] = b + lane;
     }
     if (hx) write_nh_layout<D>(t, b, e - b, nullptr, sx, cs, hx, hs, hidx, hbox);
     return;
@@ -1021,17 +1023,18 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       si[d] = rp.has_sigma ? ssig[d * cs + s] : T(0);
     }
     const uint32_t fine = query_fine<D>(g, xi, tb);
-    const uint32_t c_dc = query_class<D>(g, xi, si, rp.dc, tb);
-    const uint32_t c_dm = (qperm_nh != qperm) ? query_class<D>(g, xi, si, rp.dm, tb) : c_dc;
+    const uint32_t c_dc = class_dc ? query_class<D>(g, xi, si, rp.dc, tb) : 0u;
+    const uint32_t c_dm = class_dm ? query_class<D>(g, xi, si, rp.dm, tb) : 0u;
     qkey[s] = fine | (c_dc << 8) | (c_dm << 14);
   }
   wave_memory_sync();  // qkey of every point of the tile
   int32_t* c = cnt[w];
   wave_counting_sort<kFineBits>(b, e, nullptr, c, [&](int s) { return qkey[s] & 0xFFu; }, qfine);
   wave_memory_sync();  // qfine, the input order of the class sorts
-  wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 8) & 63u; },
-                                 qperm);
-  if (qperm_nh != qperm)
+  if (class_dc)
+    wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 8) & 63u; },
+                                   qperm);
+  if (class_dm)
     wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 14) & 63u; },
                                    qperm_nh);
   if (hx) write_nh_layout<D>(t, b, e - b, qfine, sx, cs, hx, hs, hidx, hbox);
@@ -1191,7 +1194,8 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
     int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
     const T* __restrict__ sw, const T* __restrict__ sfw, const T* __restrict__ ssig,
     const uint32_t* __restrict__ stag, const int32_t* __restrict__ qperm,
-    T* __restrict__ srho, uint64_t* __restrict__ skey, const T* __restrict__ sbox) {
+    T* __restrict__ srho, uint64_t* __restrict__ skey, const T* __restrict__ sbox,
+    uint64_t* __restrict__ hkey, uint64_t* __restrict__ hbkey) {
   constexpr int U = Unroll<D>::U;
   auto process = [&](const int item) {
     const Item it = items[item];
@@ -1325,7 +1329,25 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
       else
         run(ModeC<false, false>{});
     }
-    if (active) {
+    if (hkey) {
+      // The nearest-higher block layout takes the keys here: the lanes are the
+      // tile's points in Morton order (qperm is qfine) from a multiple of 64, so
+      // lane k's point fills layout slot k of the tile and each row of 16 lanes
+      // is one block, whose largest key a row reduction gives. Lanes past the
+      // tile's points write the padding's key 0 up to the last block.
+      static_assert(kNhBlock == 16 && kWave % kNhBlock == 0, "a block is a row of 16 lanes");
+      const int tb = toff[it.tile];
+      const int k = q - tb;
+      const int span = (toff[it.tile + 1] - tb + kNhBlock - 1) / kNhBlock * kNhBlock;
+      const uint64_t key = active ? density_key(rho, stag[s]) : 0ull;
+      const uint64_t m = row_max_u64(key);
+      if (k < span) {
+        const int64_t p = nh_first_block(tb, it.tile) * kNhBlock + k;
+        hkey[p] = key;
+        if ((k % kNhBlock) == 0) hbkey[p / kNhBlock] = m;
+      }
+      if (active) srho[s] = rho;
+    } else if (active) {
       srho[s] = rho;
       skey[s] = density_key(rho, stag[s]);
     }
@@ -1515,29 +1537,6 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
 // monotone), so the results are those of the plain scan.
 // ---------------------------------------------------------------------------
 
-// Keys of the nearest-higher layout (the ordering key of each slot's point, 0 for
-// padding: never higher) and the largest key of every block, once the density
-// stage has written the keys. One thread per slot; a block's 16 slots are 16
-// consecutive lanes, reduced by shuffles. Blocks that belong to no tile hold
-// whatever the layout left in their index slots; they are never read.
-static __global__ __launch_bounds__(kBlock) void k_nh_block_keys(const int32_t* __restrict__ hidx,
-                                                                const uint64_t* __restrict__ skey,
-                                                                int64_t n_slots, int n_points,
-                                                                uint64_t* __restrict__ hkey,
-                                                                uint64_t* __restrict__ hbkey) {
-  static_assert(kBlock % kNhBlock == 0, "blocks must not straddle workgroups");
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  uint64_t k = 0ull;
-  if (p < n_slots) {
-    const int s = hidx[p];
-    k = (s >= 0 && s < n_points) ? skey[s] : 0ull;
-    hkey[p] = k;
-  }
-  static_assert(kNhBlock == 16, "block maxima are reduced over DPP rows of 16 lanes");
-  const uint64_t m = row_max_u64(k);
-  if (p < n_slots && (p % kNhBlock) == 0) hbkey[p / kNhBlock] = m;
-}
-
 // Wave-wide minimum and maximum of floating-point values, and a lower bound of
 // the wave's smallest ordering key. (LDS permutes here: the DPP form costs more
 // vector instructions, and this scan is short of them; measured on MI355X.)
@@ -1617,8 +1616,8 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
     const int32_t* __restrict__ toff, const Item* __restrict__ items,
     int32_t* __restrict__ n_items, const T* __restrict__ sx, int64_t cs,
     const T* __restrict__ ssig, const T* __restrict__ su, const uint32_t* __restrict__ stag,
-    const T* __restrict__ srho, const uint64_t* __restrict__ skey,
-    const int32_t* __restrict__ qperm, const T* __restrict__ hx, int64_t hs,
+    const T* __restrict__ srho, const int32_t* __restrict__ qperm, const T* __restrict__ hx,
+    int64_t hs,
     const uint64_t* __restrict__ hkey, const int32_t* __restrict__ hidx,
     const T* __restrict__ hbox, const uint64_t* __restrict__ hbkey, int32_t* __restrict__ snh,
     T* __restrict__ sdelta) {
@@ -2035,7 +2034,8 @@ static void launch_density_dmk(const NeighbourLaunch<T>& L) {
   auto go = [&](auto kern, unsigned grid) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
                        b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.w, b.sorted.fw,
-                       b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey, b.sbox);
+                       b.sorted.sigma[0], b.sorted.tag, b.qperm, b.srho, b.skey, b.sbox,
+                       b.hkey, b.hbkey);
   };
   if constexpr (kPersistentBuilt<D>) {
     if (L.rp->persistent > 0) {
@@ -2079,23 +2079,13 @@ static void launch_nh_dm(const NeighbourLaunch<T>& L) {
                      b.snh, b.sdelta);
 }
 
-// Largest key of each nearest-higher layout block, once the density stage has
-// written the keys.
-template <class T>
-static void launch_nh_block_keys(const NeighbourLaunch<T>& L) {
-  const BatchBuffersT<T>& b = *L.bb;
-  const int64_t n_slots = b.h_stride;
-  hipLaunchKernelGGL(k_nh_block_keys, dim3(static_cast<unsigned>((n_slots + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, L.s, b.hidx, b.skey, n_slots, b.n_total, b.hkey, b.hbkey);
-}
-
 template <int D, int M, class T>
 static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
   auto go = [&](auto kern, unsigned grid) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(L.block), 0, L.s, *L.rp, b.events, b.geo, b.toff,
                        b.items, b.n_items, b.sorted.x[0], L.cs, b.sorted.sigma[0], b.sorted.u,
-                       b.sorted.tag, b.srho, b.skey, b.qfine, b.hx, b.h_stride, b.hkey, b.hidx,
+                       b.sorted.tag, b.srho, b.qfine, b.hx, b.h_stride, b.hkey, b.hidx,
                        b.hbox, b.hbkey, b.snh, b.sdelta);
   };
   if constexpr (kPersistentBuilt<D>) {
@@ -2112,7 +2102,6 @@ static void launch_nh_blocks_dm(const NeighbourLaunch<T>& L) {
 // blocks its lanes want, so grouping lanes by search-box class buys nothing here.
 template <int D, class T>
 static void launch_nh_blocks_d(const NeighbourLaunch<T>& L) {
-  launch_nh_block_keys(L);
   switch (L.rp->metric) {
     case M_EUCLID: launch_nh_blocks_dm<D, M_EUCLID>(L); break;
     case M_WEUCLID: launch_nh_blocks_dm<D, M_WEUCLID>(L); break;

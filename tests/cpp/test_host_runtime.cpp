@@ -92,10 +92,12 @@ static void test_workspace() {
     const double per_point = total / n;
     std::printf("workspace bytes per point, D=%d: %.1f (tiles %lld)\n", D, per_point,
                 static_cast<long long>(bp.n_tiles_total));
-    CHECK(per_point > 100.0 && per_point < 96.0 + 32.0 * D);
+    CHECK(per_point > 90.0 && per_point < 96.0 + 32.0 * D);
     // Every per-point column the stages index is there.
-    CHECK(b[S_SX] >= sizeof(float) * n * D && b[S_K0] >= 4 * n && b[S_SKEY] >= 8 * n);
-    CHECK(b[S_HX] >= sizeof(float) * D * n && b[S_HIDX] >= 4 * n);
+    CHECK(b[S_SX] >= sizeof(float) * n * D && b[S_K0] >= 4 * n);
+    CHECK(b[S_HX] >= sizeof(float) * D * n && b[S_HIDX] >= 4 * n && b[S_HKEY] >= 8 * n);
+    // With the block layout the keys live there only.
+    CHECK(b[S_SKEY] == 0);
   }
   // Optional columns only when used.
   WorkspaceShape w;
@@ -113,6 +115,11 @@ static void test_workspace() {
   w.value_bytes = w.key_bytes = 8;
   workspace_requests(w, b);
   CHECK(b[S_SSIG] == 8 * 1024 * 2 && b[S_SU] == 8000 && b[S_SFW] == 8000 && b[S_QPERM_NH] == 4000);
+  CHECK(b[S_QPERM] == 4000 && b[S_SKEY] == 8000);
+  // Density in Morton order (candidate boxes): no class order for dc.
+  w.sorted_boxes = w.density_morton = true;
+  workspace_requests(w, b);
+  CHECK(b[S_QPERM] == 0 && b[S_QFINE] == 4000 && b[S_QPERM_NH] == 4000 && b[S_SBOX] > 0);
   CHECK(b[S_EXT] == 8 * 1 * 2 * 2 && b[S_GEO] == 8 * 3 * 2);
 }
 
