@@ -968,9 +968,9 @@ __global__ __launch_bounds__(kBlock) void k_sorted_boxes(const T* __restrict__ s
 // order that aliases qfine, or qperm_nh aliasing qperm, is not sorted again.
 template <int D, class T>
 __global__ __launch_bounds__(kBlock) void k_query_order(
-    const RunParamsT<T> rp, const EventDesc* __restrict__ events, int n_events,
-    const T* __restrict__ geo, const int32_t* __restrict__ toff, int n_tiles,
-    const T* __restrict__ sx, int64_t cs, const T* __restrict__ ssig,
+    const RunParamsT<T> rp, const EventDesc* __restrict__ events, const Item* __restrict__ items,
+    const int32_t* __restrict__ ioff, const T* __restrict__ geo, const int32_t* __restrict__ toff,
+    int n_tiles, const T* __restrict__ sx, int64_t cs, const T* __restrict__ ssig,
     uint32_t* __restrict__ qkey, int32_t* __restrict__ qfine, int32_t* __restrict__ qperm,
     int32_t* __restrict__ qperm_nh, T* __restrict__ hx, int64_t hs, int32_t* __restrict__ hidx,
     T* __restrict__ hbox) {
@@ -994,17 +994,11 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
     if (hx) write_nh_layout<D>(t, b, e - b, nullptr, sx, cs, hx, hs, hidx, hbox);
     return;
   }
-  // Event of tile t: last event with t_base <= t.
-  int lo_e = 0, hi_e = n_events - 1;
-  while (lo_e < hi_e) {
-    const int mid = (lo_e + hi_e + 1) >> 1;
-    if (events[mid].t_base <= t)
-      lo_e = mid;
-    else
-      hi_e = mid - 1;
-  }
-  const EventDesc ev = events[lo_e];
-  const GeoT<T> g = event_geo(geo, rp, lo_e, ev.nper);
+  // Event of tile t: that of its first work item (ioff: the items' exclusive
+  // scan), two dependent loads where a search over the events takes a dozen.
+  const int ev_i = items[ioff[t]].event;
+  const EventDesc ev = events[ev_i];
+  const GeoT<T> g = event_geo(geo, rp, ev_i, ev.nper);
   int tb[D];
   {
     int r = t - ev.t_base;
@@ -2163,7 +2157,7 @@ static void launch_query_order_d(const NeighbourLaunch<T>& L) {
   const BatchBuffersT<T>& b = *L.bb;
   const unsigned grid = static_cast<unsigned>((b.n_tiles_total + kWaves - 1) / kWaves);
   hipLaunchKernelGGL((k_query_order<D, T>), dim3(grid), dim3(kBlock), 0, L.s, *L.rp, b.events,
-                     b.n_events, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
+                     b.items, b.icount, b.geo, b.toff, b.n_tiles_total, b.sorted.x[0], L.cs,
                      b.sorted.sigma[0], b.qkey, b.qfine, b.qperm, b.qperm_nh, b.hx, b.h_stride,
                      b.hidx, b.hbox);
 }
