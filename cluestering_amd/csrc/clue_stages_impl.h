@@ -794,36 +794,43 @@ __device__ __forceinline__ double fmin_t(double a, double b) { return __builtin_
 __device__ __forceinline__ float fmax_t(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double fmax_t(double a, double b) { return __builtin_fmax(a, b); }
 
-// Minimum and maximum over each row of 16 lanes, in every lane of the row (float
-// through DPP lane moves: pair swap, quad swap, half-row mirror, row mirror; lanes
-// a move does not write take the identity, so each move fuses into its operation).
-template <class F>
+// Minimum and maximum over each group of N = 8 or 16 lanes (a half row or a row),
+// in every lane of the group (float through DPP lane moves: pair swap, quad swap,
+// half-row mirror, then for 16 the row mirror; lanes a move does not write take
+// the identity, so each move fuses into its operation).
+template <int N, class F>
 __device__ __forceinline__ float row_reduce_dpp(float v, float id, F op) {
+  static_assert(N == 8 || N == 16, "a half row or a row");
   const int i = __float_as_int(id);
   int x = __float_as_int(v);
   x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0xB1, 0xF, 0xF, false))));
   x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x4E, 0xF, 0xF, false))));
   x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x141, 0xF, 0xF, false))));
-  x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x140, 0xF, 0xF, false))));
+  if constexpr (N == 16)
+    x = __float_as_int(op(__int_as_float(x), __int_as_float(__builtin_amdgcn_update_dpp(i, x, 0x140, 0xF, 0xF, false))));
   return __int_as_float(x);
 }
+template <int N = 16>
 __device__ __forceinline__ float row_min(float v) {
-  return row_reduce_dpp(v, __builtin_inff(), [](float a, float b) { return __builtin_fminf(a, b); });
+  return row_reduce_dpp<N>(v, __builtin_inff(), [](float a, float b) { return __builtin_fminf(a, b); });
 }
+template <int N = 16>
 __device__ __forceinline__ float row_max(float v) {
-  return row_reduce_dpp(v, -__builtin_inff(), [](float a, float b) { return __builtin_fmaxf(a, b); });
+  return row_reduce_dpp<N>(v, -__builtin_inff(), [](float a, float b) { return __builtin_fmaxf(a, b); });
 }
+template <int N = 16>
 __device__ __forceinline__ double row_min(double v) {
 #pragma unroll
-  for (int off = 1; off < 16; off <<= 1) v = fmin_t(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < N; off <<= 1) v = fmin_t(v, __shfl_xor(v, off, 64));
   return v;
 }
+template <int N = 16>
 __device__ __forceinline__ double row_max(double v) {
 #pragma unroll
-  for (int off = 1; off < 16; off <<= 1) v = fmax_t(v, __shfl_xor(v, off, 64));
+  for (int off = 1; off < N; off <<= 1) v = fmax_t(v, __shfl_xor(v, off, 64));
   return v;
 }
-// Largest 64-bit key over each row of 16 lanes (both halves moved by DPP).
+// Largest 64-bit key over each group of N = 8 or 16 lanes (both halves moved by DPP).
 template <int Ctrl>
 __device__ __forceinline__ uint64_t max_dpp_u64(uint64_t v) {
   const int lo = static_cast<int>(static_cast<uint32_t>(v));
@@ -833,11 +840,14 @@ __device__ __forceinline__ uint64_t max_dpp_u64(uint64_t v) {
   const uint64_t o = (static_cast<uint64_t>(ohi) << 32) | olo;
   return o > v ? o : v;
 }
+template <int N = 16>
 __device__ __forceinline__ uint64_t row_max_u64(uint64_t v) {
+  static_assert(N == 8 || N == 16, "a half row or a row");
   v = max_dpp_u64<0xB1>(v);   // quad_perm 1,0,3,2
   v = max_dpp_u64<0x4E>(v);   // quad_perm 2,3,0,1
   v = max_dpp_u64<0x141>(v);  // row_half_mirror
-  return max_dpp_u64<0x140>(v);  // row_mirror
+  if constexpr (N == 16) v = max_dpp_u64<0x140>(v);  // row_mirror
+  return v;
 }
 
 // First layout block of tile t whose candidates begin at sorted position b:
@@ -858,7 +868,7 @@ __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int
                                                 T* __restrict__ hx, int64_t hs,
                                                 int32_t* __restrict__ hidx, T* __restrict__ hbox) {
   static_assert(kWave % kNhBlock == 0, "blocks must not straddle waves");
-  static_assert(kNhBlock == 16, "block boxes are reduced over DPP rows of 16 lanes");
+  static_assert(kNhBlock == 8 || kNhBlock == 16, "block boxes are reduced over DPP half rows or rows");
   const int lane = lane_id();
   const int64_t blk0 = nh_first_block(b, t);
   const int span = (cnt + kNhBlock - 1) / kNhBlock * kNhBlock;
@@ -884,8 +894,8 @@ __device__ __forceinline__ void write_nh_layout(int t, int b, int cnt, const int
     }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      lo[d] = row_min(lo[d]);
-      hi[d] = row_max(hi[d]);
+      lo[d] = row_min<kNhBlock>(lo[d]);
+      hi[d] = row_max<kNhBlock>(hi[d]);
     }
     if (k < span && (k % kNhBlock) == 0) {
       const int64_t q = blk0 + k / kNhBlock;
@@ -1329,12 +1339,12 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_density(
       // lane k's point fills layout slot k of the tile and each row of 16 lanes
       // is one block, whose largest key a row reduction gives. Lanes past the
       // tile's points write the padding's key 0 up to the last block.
-      static_assert(kNhBlock == 16 && kWave % kNhBlock == 0, "a block is a row of 16 lanes");
+      static_assert(kWave % kNhBlock == 0, "a block is a group of lanes of one wave");
       const int tb = toff[it.tile];
       const int k = q - tb;
       const int span = (toff[it.tile + 1] - tb + kNhBlock - 1) / kNhBlock * kNhBlock;
       const uint64_t key = active ? density_key(rho, stag[s]) : 0ull;
-      const uint64_t m = row_max_u64(key);
+      const uint64_t m = row_max_u64<kNhBlock>(key);
       if (k < span) {
         const int64_t p = nh_first_block(tb, it.tile) * kNhBlock + k;
         hkey[p] = key;
