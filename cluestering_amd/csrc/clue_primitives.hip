@@ -112,9 +112,38 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* in, int32_t
   }
 }
 
+// One block for n <= kScanTile: the downsweep with no partials, and the total
+// (small scans, such as one event's tiles or a batch's per-event seed counts,
+// then take one launch instead of three).
+__global__ __launch_bounds__(kBlock) void k_scan_single(const int32_t* in, int32_t* out, int64_t n,
+                                                        int32_t* __restrict__ total_out) {
+  const int64_t base = static_cast<int64_t>(threadIdx.x) * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? in[i] : 0;
+    s += v[k];
+  }
+  int tot;
+  int run = block_excl_scan(s, &tot);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = tot;
+}
+
 void scan_exclusive(const int32_t* in, int32_t* out, int64_t n, int32_t* temp, int32_t* total,
                     hipStream_t s) {
   if (n <= 0) return;
+  if (n <= kScanTile) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kBlock), 0, s, in, out, n, total);
+    return;
+  }
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_reduce, dim3(static_cast<unsigned>(nb)), dim3(kBlock), 0, s, in, n, temp);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s, temp, nb, total);
