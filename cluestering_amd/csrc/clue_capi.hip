@@ -162,8 +162,19 @@ clue_status run_batch(clue_ctx* ctx, const P* p, const Pts* pts, const uint32_t*
   if (st) return st;
   BatchBuffersT<T> bb;
   ResultArraysT<T> ra;
-  st = prepare_batch(ctx->ws, bp, p, pts, res, ctx->stream, &bb, &ra);
+  // One event: its descriptors are written by a kernel (one launch where the
+  // host-to-device copies of the plan take two or three).
+  const bool single = bp.events.size() == 1 && bp.events[0].p_off == 0;
+  st = prepare_batch(ctx->ws, bp, p, pts, res, ctx->stream, &bb, &ra, !single);
   if (st) return st;
+  if (single) {
+    const int n_chunks = static_cast<int>(bp.chunks.size());
+    hipLaunchKernelGGL(k_single_event_descriptors, dim3(static_cast<unsigned>((n_chunks + 255) / 256)),
+                       dim3(256), 0, ctx->stream, bp.events[0],
+                       static_cast<EventDesc*>(ctx->ws.buf[S_EVENTS].p),
+                       bb.event_order ? static_cast<int32_t*>(ctx->ws.buf[S_EORDER].p) : nullptr,
+                       static_cast<Chunk*>(ctx->ws.buf[S_CHUNKS].p), n_chunks);
+  }
   ra.event_of_cluster = event_of_cluster;
   const RunParamsT<T> rp = make_run_params(p, pts, bp);
   StageHook hook{mark_stage, ctx};

@@ -131,7 +131,7 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
 template <class P, class Pts, class Res>
 clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
                           Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
-                          ResultArraysT<value_t<P>>* ra);
+                          ResultArraysT<value_t<P>>* ra, bool upload = true);
 
 struct ScopedDevice {
   int prev = 0;

@@ -334,13 +334,13 @@ clue_status bind_batch(void* const (&slot)[S_NUM], const BatchPlan& bp, const P*
 template <class P, class Pts, class Res>
 clue_status prepare_batch(Workspace& ws, const BatchPlan& bp, const P* p, const Pts* pts,
                           Res* res, hipStream_t s, BatchBuffersT<value_t<P>>* bb,
-                          ResultArraysT<value_t<P>>* ra) {
+                          ResultArraysT<value_t<P>>* ra, bool upload) {
   size_t need[S_NUM];
   workspace_requests(workspace_shape(bp, p, pts, res), need);
   for (int k = 0; k < S_NUM; ++k) CLUE_HIP(ws.buf[k].reserve(need[k]));
   void* slot[S_NUM];
   for (int k = 0; k < S_NUM; ++k) slot[k] = ws.buf[k].p;
-  return bind_batch(slot, bp, p, pts, res, s, bb, ra, true);
+  return bind_batch(slot, bp, p, pts, res, s, bb, ra, upload);
 }
 
 template <class P, class Pts>
@@ -385,9 +385,9 @@ template clue_status bind_batch(void* const (&)[S_NUM], const BatchPlan&, const 
                                 BatchBuffersT<double>*, ResultArraysT<double>*, bool);
 template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params*,
                                    const clue_points*, clue_results*, hipStream_t,
-                                   BatchBuffersT<float>*, ResultArraysT<float>*);
+                                   BatchBuffersT<float>*, ResultArraysT<float>*, bool);
 template clue_status prepare_batch(Workspace&, const BatchPlan&, const clue_params_f64*,
                                    const clue_points_f64*, clue_results_f64*, hipStream_t,
-                                   BatchBuffersT<double>*, ResultArraysT<double>*);
+                                   BatchBuffersT<double>*, ResultArraysT<double>*, bool);
 
 }  // namespace clue_rt
