@@ -1846,7 +1846,7 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // the results are then written in input order, each point's sorted values
 // gathered through iperm, so that every store is coalesced.
 // ---------------------------------------------------------------------------
-constexpr int kLinkJumpPasses = 3;  // pointer-jumping passes before the label chase
+constexpr int kLinkJumpPasses = 4;  // pointer-jumping passes before the label chase
 constexpr int kEvtBitWords = kEvtMaxPoints / 32;
 static_assert(kEvtBitWords % kEvtBlock == 0, "bit words per thread");
 
