@@ -27,6 +27,38 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Inclusive scan within a wave.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off, 64);
+    if (l >= off) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan across the Block threads of a workgroup; the total in *total.
+template <int Block>
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+  constexpr int kWaves = Block / kWave;
+  __shared__ int wsum[kWaves];
+  const int w = threadIdx.x / kWave;
+  const int incl = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  int before = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) {
+    const int s = wsum[k];
+    before += (k < w) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return before + incl - v;
+}
+
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));

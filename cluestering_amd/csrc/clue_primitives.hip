@@ -19,34 +19,8 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return v;
 }
 
-// Inclusive scan within a wave.
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(v, off, 64);
-    if (l >= off) v += t;
-  }
-  return v;
-}
-
-// Exclusive scan across the 256 threads of a block; returns the block total in *total.
 __device__ __forceinline__ int block_excl_scan(int v, int* total) {
-  __shared__ int wsum[kWavesPerBlock];
-  const int w = threadIdx.x / kWave;
-  const int incl = wave_incl_scan(v);
-  if (lane_id() == kWave - 1) wsum[w] = incl;
-  __syncthreads();
-  int before = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < kWavesPerBlock; ++k) {
-    const int s = wsum[k];
-    before += (k < w) ? s : 0;
-    tot += s;
-  }
-  __syncthreads();
-  *total = tot;
-  return before + incl - v;
+  return block_excl_scan<kBlock>(v, total);
 }
 
 // ---------------------------------------------------------------------------

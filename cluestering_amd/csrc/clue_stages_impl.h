@@ -513,6 +513,48 @@ static __global__ void k_item_fill(const int32_t* __restrict__ toff, const int32
   }
 }
 
+// The three item kernels above in one workgroup, for batches of at most
+// kItemsOneBlockTiles tiles (one launch instead of five): the tiles' item counts
+// and their exclusive scan (kBlock tiles at a time, with a carry), the total and
+// the queue counters, then the items. Each thread fills the tiles whose offsets it
+// wrote, so the fill reads only its own stores.
+constexpr int kItemsOneBlockTiles = 4096;
+static __global__ __launch_bounds__(kBlock) void k_items_one_block(
+    const int32_t* __restrict__ toff, int n_tiles, const EventDesc* __restrict__ events,
+    int n_events, int32_t* __restrict__ ioff, Item* __restrict__ items, int32_t* __restrict__ ctl) {
+  int carry = 0;
+  for (int base = 0; base < n_tiles; base += kBlock) {
+    const int t = base + threadIdx.x;
+    const int c = t < n_tiles ? (toff[t + 1] - toff[t] + kWave - 1) / kWave : 0;
+    int tot;
+    const int ex = block_excl_scan<kBlock>(c, &tot);
+    if (t < n_tiles) ioff[t] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) ctl[0] = carry;
+  for (int k = kQueueDensity + threadIdx.x; k < kQueueWords; k += kBlock) ctl[k] = 0;
+  for (int t = threadIdx.x; t < n_tiles; t += kBlock) {
+    int lo = 0, hi = n_events - 1;  // event of tile t: last event with t_base <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (events[mid].t_base <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int b = toff[t], e = toff[t + 1];
+    int o = ioff[t];
+    for (int s = b; s < e; s += kWave) {
+      Item it;
+      it.tile = t;
+      it.begin = s;
+      it.end = min(e, s + kWave);
+      it.event = lo;
+      items[o++] = it;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Distances (DistanceMetrics.hpp), right fold over dimensions
 // (internal/meta/accumulate.hpp): t0 + (t1 + (... + (t_{D-1} + 0))).
@@ -1856,7 +1898,8 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
     const T* __restrict__ srho, int32_t* __restrict__ snh, const T* __restrict__ sdelta,
     const T* __restrict__ su, T* __restrict__ rho, int32_t* __restrict__ nh, T* __restrict__ delta,
-    int32_t* __restrict__ is_seed, int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds) {
+    int32_t* __restrict__ is_seed, int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds,
+    int32_t* __restrict__ lone_offsets, int32_t* __restrict__ lone_total) {
   constexpr int kWordsPerThread = kEvtBitWords / kEvtBlock;
   __shared__ uint32_t bits[kEvtBitWords];
   __shared__ int32_t wsum[kEvtWaves];
@@ -1922,7 +1965,13 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
       rank[b0 + (w0 + k) * 32 + bit] = run++;
     }
   }
-  if (threadIdx.x == 0) event_seeds[e] = total;
+  if (threadIdx.x == 0) {
+    event_seeds[e] = total;
+    if (lone_total) {  // a batch of one event: the scan over events is the identity
+      lone_offsets[0] = 0;
+      *lone_total = total;
+    }
+  }
   // Pointer jumping on the event's links (as k_link_jump; any interleaving of the
   // updates leaves every link on its own chain).
   for (int pass = 0; pass < kLinkJumpPasses; ++pass) {
@@ -2273,15 +2322,20 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   hipLaunchKernelGGL(k_gather<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ga, rp, skeys,
                      sperm, n, bb.n_tiles_total, bb.toff);
   }
-  hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                     s, bb.toff, bb.n_tiles_total, bb.icount);
-  scan_exclusive(bb.icount, bb.icount, bb.n_tiles_total, bb.scan_tmp, bb.n_items, s);
-  hipLaunchKernelGGL(k_item_fill,
-                     dim3(static_cast<unsigned>(
-                         (std::max<int64_t>(bb.n_tiles_total, kQueueWords) + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0,
-                     s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items,
-                     bb.n_items);
+  if (bb.n_tiles_total <= kItemsOneBlockTiles) {
+    hipLaunchKernelGGL(k_items_one_block, dim3(1), dim3(kBlock), 0, s, bb.toff, bb.n_tiles_total,
+                       bb.events, bb.n_events, bb.icount, bb.items, bb.n_items);
+  } else {
+    hipLaunchKernelGGL(k_item_count, dim3((bb.n_tiles_total + kBlock - 1) / kBlock), dim3(kBlock),
+                       0, s, bb.toff, bb.n_tiles_total, bb.icount);
+    scan_exclusive(bb.icount, bb.icount, bb.n_tiles_total, bb.scan_tmp, bb.n_items, s);
+    hipLaunchKernelGGL(k_item_fill,
+                       dim3(static_cast<unsigned>(
+                           (std::max<int64_t>(bb.n_tiles_total, kQueueWords) + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0,
+                       s, bb.toff, bb.icount, bb.n_tiles_total, bb.events, bb.n_events, bb.items,
+                       bb.n_items);
+  }
 
   NeighbourLaunch<T> L;
   L.rp = &rp;
@@ -2308,11 +2362,12 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   // (d), (e)
   mark(kStageSeeds);
   if (bb.event_order) {
+    const bool lone = bb.n_events == 1;
     hipLaunchKernelGGL(k_event_seeds<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock),
                        0, s, rp, bb.events, bb.event_order, bb.sorted.perm, bb.iperm, bb.srho, bb.snh,
                        bb.sdelta, bb.sorted.u, out.rho, out.nh, out.delta, out.is_seed, bb.rank,
-                       bb.event_seeds);
-    scan_exclusive(bb.event_seeds, bb.event_offsets, bb.n_events, bb.scan_tmp, bb.n_seeds, s);
+                       bb.event_seeds, lone ? bb.event_offsets : nullptr, lone ? bb.n_seeds : nullptr);
+    if (!lone) scan_exclusive(bb.event_seeds, bb.event_offsets, bb.n_events, bb.scan_tmp, bb.n_seeds, s);
     mark(kStageLabels);
   } else {
     hipLaunchKernelGGL(k_unpermute<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, rp, n,
