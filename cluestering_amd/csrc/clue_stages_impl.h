@@ -776,19 +776,22 @@ __device__ __forceinline__ void wave_memory_sync() {
 // Stable counting sort of the points at positions [b, e) of `order` (identity
 // when NULL) by a Bits-bit key, into out[b, e). One wave; cnt is its LDS
 // counter array of 2^Bits entries. The counts need no order (one LDS atomic per
-// point); the placement visits rounds and lanes in position order and ranks equal
+// point; with Counted the caller has made them already, while it computed the
+// keys); the placement visits rounds and lanes in position order and ranks equal
 // keys by ballot, so equal keys keep their input order. Placement takes four
 // rounds' positions and keys with their loads in flight together, then ranks
 // the rounds one after the other.
-template <int Bits, class KeyOf>
+template <int Bits, bool Counted = false, class KeyOf>
 __device__ __forceinline__ void wave_counting_sort(int b, int e, const int32_t* order,
                                                    int32_t* cnt, KeyOf key_of, int32_t* out) {
   constexpr int kKeys = 1 << Bits;
   constexpr int kRounds = 4;
   const int lane = lane_id();
-  for (int c = lane; c < kKeys; c += kWave) cnt[c] = 0;
-  __builtin_amdgcn_wave_barrier();
-  for (int i = b + lane; i < e; i += kWave) atomicAdd(&cnt[key_of(order ? order[i] : i)], 1);
+  if constexpr (!Counted) {
+    for (int c = lane; c < kKeys; c += kWave) cnt[c] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = b + lane; i < e; i += kWave) atomicAdd(&cnt[key_of(order ? order[i] : i)], 1);
+  }
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
   int carry = b;
@@ -1060,6 +1063,10 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
       r /= ev.nper;
     }
   }
+  // The keys, and the Morton sort's counts as they are made.
+  int32_t* c = cnt[w];
+  for (int k = lane; k < (1 << kFineBits); k += kWave) c[k] = 0;
+  __builtin_amdgcn_wave_barrier();
 #pragma unroll 4
   for (int s = b + lane; s < e; s += kWave) {
     T xi[D], si[D];
@@ -1072,10 +1079,11 @@ __global__ __launch_bounds__(kBlock) void k_query_order(
     const uint32_t c_dc = class_dc ? query_class<D>(g, xi, si, rp.dc, tb) : 0u;
     const uint32_t c_dm = class_dm ? query_class<D>(g, xi, si, rp.dm, tb) : 0u;
     qkey[s] = fine | (c_dc << 8) | (c_dm << 14);
+    atomicAdd(&c[fine], 1);
   }
   wave_memory_sync();  // qkey of every point of the tile
-  int32_t* c = cnt[w];
-  wave_counting_sort<kFineBits>(b, e, nullptr, c, [&](int s) { return qkey[s] & 0xFFu; }, qfine);
+  wave_counting_sort<kFineBits, true>(b, e, nullptr, c, [&](int s) { return qkey[s] & 0xFFu; },
+                                      qfine);
   wave_memory_sync();  // qfine, the input order of the class sorts
   if (class_dc)
     wave_counting_sort<kClassBits>(b, e, qfine, c, [&](int s) { return (qkey[s] >> 8) & 63u; },
