@@ -1894,11 +1894,22 @@ __global__ __launch_bounds__(kBlock) void k_unpermute(
 // The seeds of earlier events are added by the label stage from an exclusive
 // scan of event_seeds. The seed flags and links are settled in sorted order;
 // the results are then written in input order, each point's sorted values
-// gathered through iperm, so that every store is coalesced.
+// gathered through iperm, so that every store is coalesced. An event of at
+// most kEvtLdsLinks points keeps its links in LDS as 16-bit event positions:
+// the results pass reads them there, the jumping passes run there (more of
+// them, as they cost no memory traffic) and the links go back to memory once.
 // ---------------------------------------------------------------------------
-constexpr int kLinkJumpPasses = 4;  // pointer-jumping passes before the label chase
+constexpr int kLinkJumpPasses = 4;     // pointer-jumping passes in memory before the label chase
+constexpr int kLdsJumpPasses = 6;      // in LDS: chains of up to 64 links end at their root
+constexpr int kEvtLdsLinks = 1 << 15;  // events whose links fit LDS (16-bit positions)
 constexpr int kEvtBitWords = kEvtMaxPoints / 32;
+constexpr int kLdsBitWords = kEvtLdsLinks / 32;
+// One LDS pool: the seed bits of any event, or the bits and links of a small one
+// (two workgroups per CU, as the 16 waves of each allow).
+constexpr int kSeedPoolWords =
+    kEvtBitWords > kLdsBitWords + kEvtLdsLinks / 2 ? kEvtBitWords : kLdsBitWords + kEvtLdsLinks / 2;
 static_assert(kEvtBitWords % kEvtBlock == 0, "bit words per thread");
+static_assert(kEvtLdsLinks <= 32768, "links are 16-bit event positions");
 
 template <class T>
 __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
@@ -1909,14 +1920,18 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     int32_t* __restrict__ is_seed, int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds,
     int32_t* __restrict__ lone_offsets, int32_t* __restrict__ lone_total) {
   constexpr int kWordsPerThread = kEvtBitWords / kEvtBlock;
-  __shared__ uint32_t bits[kEvtBitWords];
+  __shared__ uint32_t pool[kSeedPoolWords];
   __shared__ int32_t wsum[kEvtWaves];
+  uint32_t* bits = pool;
+  int16_t* links = reinterpret_cast<int16_t*>(pool + kLdsBitWords);
   const int e = order[blockIdx.x];
   const EventDesc ev = events[e];
   const int b0 = ev.p_off, n = ev.n;
+  const bool in_lds = n <= kEvtLdsLinks;
+  const int n_words = in_lds ? kLdsBitWords : kEvtBitWords;  // bit words in use
   const int w = threadIdx.x / kWave;
   const int lane = lane_id();
-  for (int k = threadIdx.x; k < kEvtBitWords; k += kEvtBlock) bits[k] = 0u;
+  for (int k = threadIdx.x; k < n_words; k += kEvtBlock) bits[k] = 0u;
   __syncthreads();
   // Seed flags (ClusteringKernels.hpp:300-316) in sorted order: a seed's link
   // becomes -2, and its bit (input order) is set.
@@ -1926,8 +1941,9 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     const int h = snh[s];
     const T u = rp.has_u ? su[s] : T(1);
     const bool seed = (h == -1) && (srho[s] >= rp.rhoc * u);
+    if (in_lds) links[k] = static_cast<int16_t>(seed ? -2 : (h >= 0 ? h - b0 : h));
     if (seed) {
-      snh[s] = -2;
+      if (!in_lds) snh[s] = -2;
       const int i = perm[s] - b0;
       atomicOr(&bits[i >> 5], 1u << (i & 31));
     }
@@ -1939,17 +1955,24 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
   for (int k = threadIdx.x; k < n; k += kEvtBlock) {
     const int i = b0 + k;
     const int s = iperm[i];
-    const int h = snh[s];
+    int h;
+    if (in_lds) {
+      h = links[s - b0];
+      h = h >= 0 ? h + b0 : h;
+    } else {
+      h = snh[s];
+    }
     rho[i] = srho[s];
     nh[i] = h >= 0 ? perm[h] : -1;
     if (delta) delta[i] = sdelta[s];
     is_seed[i] = h == -2 ? 1 : 0;
   }
-  // Ranks in input order: each thread owns kWordsPerThread consecutive words.
+  // Ranks in input order: each thread owns kWordsPerThread consecutive words
+  // (words past the bits in use count nothing).
   const int w0 = threadIdx.x * kWordsPerThread;
   int mine = 0;
 #pragma unroll
-  for (int k = 0; k < kWordsPerThread; ++k) mine += __popc(bits[w0 + k]);
+  for (int k = 0; k < kWordsPerThread; ++k) mine += w0 + k < n_words ? __popc(bits[w0 + k]) : 0;
   int incl = mine;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
@@ -1966,7 +1989,7 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
   int run = before + incl - mine;
 #pragma unroll
   for (int k = 0; k < kWordsPerThread; ++k) {
-    uint32_t word = bits[w0 + k];
+    uint32_t word = w0 + k < n_words ? bits[w0 + k] : 0u;
     while (word) {
       const int bit = __builtin_ctz(word);
       word &= word - 1u;
@@ -1982,15 +2005,32 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
   }
   // Pointer jumping on the event's links (as k_link_jump; any interleaving of the
   // updates leaves every link on its own chain).
-  for (int pass = 0; pass < kLinkJumpPasses; ++pass) {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  if (in_lds) {
+    for (int pass = 0; pass < kLdsJumpPasses; ++pass) {
+      __syncthreads();
+      for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+        const int l = links[k];
+        if (l < 0) continue;
+        const int l2 = links[l];
+        if (l2 >= 0) links[k] = static_cast<int16_t>(l2);
+      }
+    }
     __syncthreads();
     for (int k = threadIdx.x; k < n; k += kEvtBlock) {
-      const int s = b0 + k;
-      const int l = snh[s];
-      if (l < 0) continue;
-      const int l2 = snh[l];
-      if (l2 >= 0) snh[s] = l2;
+      const int l = links[k];
+      snh[b0 + k] = l >= 0 ? l + b0 : l;
+    }
+  } else {
+    for (int pass = 0; pass < kLinkJumpPasses; ++pass) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __syncthreads();
+      for (int k = threadIdx.x; k < n; k += kEvtBlock) {
+        const int s = b0 + k;
+        const int l = snh[s];
+        if (l < 0) continue;
+        const int l2 = snh[l];
+        if (l2 >= 0) snh[s] = l2;
+      }
     }
   }
 }
