@@ -168,8 +168,9 @@ def main():
                     help="only run a plain device copy of MIB MiB (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus > 1) are not "
-              "supported (see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: the benchmark runs on one GPU; multi-GPU runs (--gpus > 1) are not "
+              "supported (events would shard across GPUs by whole events, one graph each).",
+              file=sys.stderr)
         sys.exit(2)
 
     from cluestering_amd.device import Clusterer, Context, Plan, PointsDevice
