@@ -16,16 +16,16 @@ struct TuningEntry {
 };
 
 inline constexpr TuningEntry kTuningTable[] = {
-    {2, 2048, 0, 8, 1, 0},
+    {2, 2048, 1, 6, 1, 0},
     {2, 16384, 1, 4, 1, 0},
-    {2, 131072, 1, 4, 1, 0},
-    {2, INT64_MAX, 1, -1, 1, 0},
+    {2, 131072, 1, 6, 1, 0},
+    {2, INT64_MAX, 1, 4, 2, 0},
     {3, 2048, 1, 8, 1, 0},
-    {3, 16384, 1, 6, 1, 0},
+    {3, 16384, 1, 8, 1, 0},
     {3, 131072, 1, 8, 1, 0},
-    {3, INT64_MAX, 1, 6, 1, 0},
+    {3, INT64_MAX, 1, 8, 2, 0},
     {4, 2048, 1, 8, 1, 0},
-    {4, 16384, 1, 6, 1, 0},
+    {4, 16384, 1, 4, 1, 0},
     {4, INT64_MAX, 1, 6, 1, 0},
     {6, 2048, 0, 6, 1, 0},
     {6, 16384, 0, 8, 1, 0},
