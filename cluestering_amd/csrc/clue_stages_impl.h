@@ -1918,7 +1918,8 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     const T* __restrict__ srho, int32_t* __restrict__ snh, const T* __restrict__ sdelta,
     const T* __restrict__ su, T* __restrict__ rho, int32_t* __restrict__ nh, T* __restrict__ delta,
     int32_t* __restrict__ is_seed, int32_t* __restrict__ rank, int32_t* __restrict__ event_seeds,
-    int32_t* __restrict__ lone_offsets, int32_t* __restrict__ lone_total) {
+    int32_t* __restrict__ lone_offsets, int32_t* __restrict__ lone_total,
+    int32_t* __restrict__ cluster) {
   constexpr int kWordsPerThread = kEvtBitWords / kEvtBlock;
   __shared__ uint32_t pool[kSeedPoolWords];
   __shared__ int32_t wsum[kEvtWaves];
@@ -1993,7 +1994,16 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
     while (word) {
       const int bit = __builtin_ctz(word);
       word &= word - 1u;
-      rank[b0 + (w0 + k) * 32 + bit] = run++;
+      const int ii = (w0 + k) * 32 + bit;  // the seed's input position in the event
+      if (in_lds) {
+        // The event's seed list in rank order, and the seed's link: its rank as
+        // a code (-3 - rank) that the jumping passes carry to its followers.
+        rank[b0 + run] = b0 + ii;
+        links[iperm[b0 + ii] - b0] = static_cast<int16_t>(-3 - run);
+        ++run;
+      } else {
+        rank[b0 + ii] = run++;
+      }
     }
   }
   if (threadIdx.x == 0) {
@@ -2006,19 +2016,26 @@ __global__ __launch_bounds__(kEvtBlock) void k_event_seeds(
   // Pointer jumping on the event's links (as k_link_jump; any interleaving of the
   // updates leaves every link on its own chain).
   if (in_lds) {
+    // A link is a position, a seed's code or -1 (the end of an outlier chain), and
+    // every point shares the label of what its link names; replacing a position
+    // by that point's link keeps this, in any interleaving. After the passes the
+    // few chains longer than 2^kLdsJumpPasses are finished by a chase in LDS, and
+    // each point's label within the event goes to cluster[] in input order (the
+    // label stage adds the event's offset).
     for (int pass = 0; pass < kLdsJumpPasses; ++pass) {
       __syncthreads();
       for (int k = threadIdx.x; k < n; k += kEvtBlock) {
         const int l = links[k];
-        if (l < 0) continue;
-        const int l2 = links[l];
-        if (l2 >= 0) links[k] = static_cast<int16_t>(l2);
+        if (l >= 0) links[k] = links[l];
       }
     }
     __syncthreads();
+#pragma unroll 4
     for (int k = threadIdx.x; k < n; k += kEvtBlock) {
-      const int l = links[k];
-      snh[b0 + k] = l >= 0 ? l + b0 : l;
+      const int i = b0 + k;
+      int l = links[iperm[i] - b0];
+      while (l >= 0) l = links[l];
+      cluster[i] = l <= -3 ? -3 - l : -1;
     }
   } else {
     for (int pass = 0; pass < kLinkJumpPasses; ++pass) {
@@ -2062,20 +2079,38 @@ static __global__ __launch_bounds__(kBlock) void k_link_jump(int32_t* __restrict
 // chased from sorted positions, so their first links are read in order and the
 // label stores scatter within each event (chasing from input order instead
 // stores in order but reads twice the bytes, for the same time on MI355X).
+// An event the fused seed stage resolved in LDS (at most kEvtLdsLinks points)
+// has its labels within the event in cluster[] already and its seed list in
+// rank[p_off, p_off + event_seeds[e]): only the offset is added.
 static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restrict__ chunks,
                                                    const EventDesc* __restrict__ events,
                                                    const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ link,
                                                    const int32_t* __restrict__ rank,
                                                    const int32_t* __restrict__ event_offsets,
+                                                   const int32_t* __restrict__ event_seeds,
                                                    int32_t* __restrict__ cluster,
                                                    int32_t* __restrict__ seeds,
                                                    int32_t* __restrict__ event_of_cluster,
                                                    int local_ids) {
   const Chunk ch = chunks[blockIdx.x];
+  const EventDesc ev = events[ch.event];
   // Seeds of the earlier events: added to local ranks, subtracted from global
   // ranks with per-event numbering.
-  const int first = event_offsets ? event_offsets[ch.event] : rank[events[ch.event].p_off];
+  const int first = event_offsets ? event_offsets[ch.event] : rank[ev.p_off];
+  if (event_offsets && ev.n <= kEvtLdsLinks) {
+    const int count = event_seeds[ch.event];
+    for (int p = ch.begin + threadIdx.x; p < ch.end; p += kBlock) {
+      const int c = cluster[p];
+      if (c >= 0 && !local_ids) cluster[p] = c + first;
+      const int r = p - ev.p_off;
+      if (r < count) {
+        seeds[first + r] = rank[p];
+        if (event_of_cluster) event_of_cluster[first + r] = ev.index;
+      }
+    }
+    return;
+  }
   const int add = event_offsets ? first : 0;
   const int sub = local_ids ? first : 0;
   for (int s = ch.begin + threadIdx.x; s < ch.end; s += kBlock) {
@@ -2090,7 +2125,7 @@ static __global__ __launch_bounds__(kBlock) void k_labels(const Chunk* __restric
     cluster[i] = c >= 0 ? c - sub : -1;
     if (cur == s && nx == -2) {
       seeds[c] = i;
-      if (event_of_cluster) event_of_cluster[c] = events[ch.event].index;
+      if (event_of_cluster) event_of_cluster[c] = ev.index;
     }
   }
 }
@@ -2414,7 +2449,8 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
     hipLaunchKernelGGL(k_event_seeds<T>, dim3(static_cast<unsigned>(bb.n_events)), dim3(kEvtBlock),
                        0, s, rp, bb.events, bb.event_order, bb.sorted.perm, bb.iperm, bb.srho, bb.snh,
                        bb.sdelta, bb.sorted.u, out.rho, out.nh, out.delta, out.is_seed, bb.rank,
-                       bb.event_seeds, lone ? bb.event_offsets : nullptr, lone ? bb.n_seeds : nullptr);
+                       bb.event_seeds, lone ? bb.event_offsets : nullptr, lone ? bb.n_seeds : nullptr,
+                       out.cluster);
     if (!lone) scan_exclusive(bb.event_seeds, bb.event_offsets, bb.n_events, bb.scan_tmp, bb.n_seeds, s);
     mark(kStageLabels);
   } else {
@@ -2428,7 +2464,7 @@ void launch_pipeline(const RunParamsT<T>& rp, const PointArraysT<T>& in, const B
   }
   hipLaunchKernelGGL(k_labels, dim3(bb.n_chunks), dim3(kBlock), 0, s, bb.chunks, bb.events,
                      bb.sorted.perm, bb.snh, bb.rank,
-                     bb.event_order ? bb.event_offsets : nullptr,
+                     bb.event_order ? bb.event_offsets : nullptr, bb.event_seeds,
                      out.cluster, bb.seeds, out.event_of_cluster, out.local_ids);
   mark(kNumStages);
 }
