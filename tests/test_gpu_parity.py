@@ -212,6 +212,32 @@ def test_batched_with_empty_and_tiny_events(gpu_ctx, oracle_mod):
     np.testing.assert_array_equal(eoc.to_host()[:k], ref.event_of_cluster)
 
 
+def test_batched_event_size_boundaries(gpu_ctx, oracle_mod):
+    """Events on both sides of the seed stage's LDS limit (at most 32,768 points:
+    links and labels resolved in LDS; above: jumping passes in memory and the label
+    chase), with the event of every cluster, and lone events on both sides of the
+    8,192 points up to which a lone event takes the fused per-event stages."""
+    from cluestering_amd.device import Clusterer, PointsDevice
+    rng = np.random.default_rng(17)
+    sizes = np.array([32768, 32769, 45000, 100, 32767], np.uint32)
+    parts = [synthetic._event2d(rng, int(n)) for n in sizes]
+    c = np.ascontiguousarray(np.concatenate([p[0] for p in parts], axis=1))
+    w = np.ascontiguousarray(np.concatenate([p[1] for p in parts]))
+    kern = K.GaussianKernel(1.0, 1.0)
+    _, ref = _check(Case(c, w, 1.5, 5.0, 2.0, kernel=kern, sizes=sizes, name="lds-boundary"),
+                    gpu_ctx, oracle_mod)
+    pts = PointsDevice.from_host(gpu_ctx, c, w)
+    eoc = gpu_ctx.empty(w.size, np.int32)
+    k = Clusterer(gpu_ctx, 1.5, 5.0, 2.0).make_clusters_batched(pts, sizes, None, kern,
+                                                              event_of_cluster=eoc)
+    assert k == ref.n_seeds
+    np.testing.assert_array_equal(eoc.to_host()[:k], ref.event_of_cluster)
+    for n in (8192, 8193):
+        cn, wn = synthetic._event2d(rng, n)
+        _check(Case(np.ascontiguousarray(cn), np.ascontiguousarray(wn), 1.5, 5.0, 2.0, kernel=kern,
+                    name=f"lone-{n}"), gpu_ctx, oracle_mod)
+
+
 def test_batched_reference_fixture(gpu_ctx, oracle_mod):
     # tests/test_batched_clustering.cpp:8-65: 10 copies of data_1024 give 10x the clusters.
     c, w = read_points("data_1024.csv")
