@@ -1715,13 +1715,14 @@ __attribute__((amdgpu_waves_per_eu(D <= 2 ? 8 : 1))) void k_nh_blocks(
     uint32_t tag_nh = 0u;
     T bound = eff_thr;  // prune_threshold(eff_thr, delta)
 
-    // The reference's acceptance test (ClusteringKernels.hpp:180-195).
+    // The reference's acceptance test (ClusteringKernels.hpp:180-195). Its radius
+    // test, dist <= eff, holds for every candidate that reaches it: callers pass
+    // only v <= bound <= eff_thr, the largest value whose distance is <= eff.
     auto consider = [&](T v, T rj, uint32_t tj, int64_t slot) {
       const T dist = finish_distance<M>(v);
-      if (dist <= eff &&
-          ((dist < delta) ||
-           ((dist == delta) && (nh_slot >= 0) &&
-            ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+      if ((dist < delta) ||
+          ((dist == delta) && (nh_slot >= 0) &&
+           ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh))))) {
         delta = dist;
         nh_slot = slot;
         rho_nh = rj;
