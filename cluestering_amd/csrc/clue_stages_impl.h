@@ -1518,12 +1518,13 @@ __global__ __launch_bounds__(kMaxNeighbourBlock) void k_nearest_higher(
   uint32_t tag_nh = 0u;
   T bound = eff_thr;  // prune_threshold(eff_thr, delta)
 
-  // The reference's acceptance test, in candidate order.
+  // The reference's acceptance test, in candidate order. Its radius test, dist <=
+  // eff, holds for every candidate that reaches it (callers pass only v <= bound
+  // <= eff_thr, the largest value whose distance is <= eff), so it is left out.
   auto consider = [&](T v, T rj, uint32_t tj, int j) {
     const T dist = finish_distance<M>(v);
-    if (dist <= eff &&
-        ((dist < delta) ||
-         ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh)))))) {
+    if ((dist < delta) ||
+        ((dist == delta) && (nh >= 0) && ((rj > rho_nh) || ((rj == rho_nh) && (tj > tag_nh))))) {
       delta = dist;
       nh = j;
       rho_nh = rj;
